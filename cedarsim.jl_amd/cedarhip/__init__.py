@@ -1,0 +1,16 @@
+"""cedarhip — MI355X (gfx950) Newton-transient engine for CedarSim circuits.
+
+The circuit DSL mirrors CedarSim's (src/simulate_ir.jl, src/simpledevices.jl);
+analyses mirror tran!/dc!/CircuitSweep (src/sweeps.jl).  All numerics run in
+libcedarhip.so on the GPU.
+"""
+from .netlist import (C, CedarError, CurrentSource, FlatCircuit, Gnd, I, L, Named, Net, R,
+                      SimpleCapacitor, SimpleInductor, SimpleResistor, SquareLawMOSFET,
+                      UnimplementedDevice, V, VoltageSource, Waveform, lower, net, pulse, pwl,
+                      spsin, vccs, vcvs)
+from .engine import (Batch, BatchResult, CedarHIPError, CircuitSweep, CircuitSystem, DCSolution,
+                     Engine, Solution, TranOptions, dc, default_engine, grid_options, tran,
+                     tran_batch)
+from . import lib
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,390 @@
+"""Host-side mirror of CedarSim's analysis interface on top of the C-ABI.
+
+    reference (Julia)                              here (Python)
+    ---------------------------------------------  ------------------------------------
+    CircuitIRODESystem(circ)                       CircuitSystem(engine, lower(circ))
+    prob.f.f(out, du, u, p, t)                     sys.f(out, du, u, p, t)
+    prob.f.jac(J, du, u, p, gamma, t)              sys.jac(vals, du, u, p, gamma, t)
+    jac_prototype                                  sys.jac_prototype (scipy CSR)
+    dc!(circ)            src/sweeps.jl:437-448      dc(circ)
+    tran!(circ, tspan)   src/sweeps.jl:450-465      tran(circ, tspan; reltol, abstol)
+    CircuitSweep + tran!. src/sweeps.jl:390-501     tran_batch(...) / CircuitSweep
+    sol[sys.node]                                  sol["node"]
+    sol.retcode                                    sol.retcode ("Success", ...)
+
+Every compute call runs on the GPU through libcedarhip.so; there is no CPU
+path.  Errors follow the reference's conventions: malformed circuits raise
+(CedarError / CedarHIPError), solver outcomes are reported as a retcode.
+"""
+from __future__ import annotations
+
+import math
+from ctypes import byref, c_double, c_int32, c_void_p
+from dataclasses import dataclass, fields
+from typing import Callable, Dict, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import lib as L
+from .netlist import FlatCircuit, lower
+
+
+class CedarHIPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{L.RETCODE_NAMES.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+@dataclass
+class TranOptions:
+    """Transient options (cs_tran_opts).  Defaults follow the north-star
+    accuracy (reltol 1e-6, abstol 1e-9 V) and CedarDCOp's abstol 1e-10."""
+
+    t_start: float = 0.0
+    t_stop: float = 1e-9
+    reltol: float = 1e-6
+    abstol: float = 1e-9
+    dt_init: float = 0.0
+    dt_min: float = 0.0
+    dt_max: float = 0.0
+    newton_tol: float = 1e-2
+    dt_out: float = 0.0
+    dc_abstol: float = 1e-10
+    n_out: int = 0
+    max_order: int = 2
+    newton_maxit: int = 8
+    init: int = L.INIT_DCOP
+    max_steps: int = 10_000_000
+    lanes: int = 0
+    record_steps: int = 0
+
+    def to_c(self) -> L.cs_tran_opts:
+        o = L.cs_tran_opts()
+        for f in fields(self):
+            setattr(o, f.name, getattr(self, f.name))
+        return o
+
+
+def grid_options(tspan, n_out=None, dt_out=None, **kw) -> TranOptions:
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    if dt_out is None:
+        n_out = 201 if n_out is None else int(n_out)
+        dt_out = (t1 - t0) / (n_out - 1)
+    elif n_out is None:
+        n_out = int(math.floor((t1 - t0) / dt_out + 1e-9)) + 1
+    return TranOptions(t_start=t0, t_stop=t1, dt_out=float(dt_out), n_out=int(n_out), **kw)
+
+
+class Engine:
+    """One HIP device context (cs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load_library()
+        ctx = c_void_p()
+        rc = self.lib.cs_create(int(device), byref(ctx))
+        if rc != L.CS_OK:
+            raise CedarHIPError(rc, "cs_create failed (no HIP device?)")
+        self.ctx = ctx
+        self._systems: Dict[int, "CircuitSystem"] = {}
+
+    def check(self, rc: int):
+        if rc != L.CS_OK:
+            msg = self.lib.cs_last_error(self.ctx)
+            raise CedarHIPError(rc, msg.decode() if msg else "")
+
+    def load(self, circuit) -> "CircuitSystem":
+        flat = circuit if isinstance(circuit, FlatCircuit) else lower(circuit)
+        return CircuitSystem(self, flat)
+
+    def close(self):
+        if self.ctx:
+            self.lib.cs_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_engine: Optional[Engine] = None
+
+
+def default_engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+class CircuitSystem:
+    """The compiled circuit: symbolic plan on the device plus the DAE interface."""
+
+    def __init__(self, engine: Engine, flat: FlatCircuit):
+        self.engine = engine
+        self.flat = flat
+        self._nl = L.NetlistStruct(flat)
+        cid = c_int32()
+        engine.check(engine.lib.cs_circuit_load(engine.ctx, byref(self._nl.struct), byref(cid)))
+        self.cid = cid.value
+        info = L.cs_circuit_info()
+        engine.check(engine.lib.cs_circuit_info_get(engine.ctx, self.cid, byref(info)))
+        self.info = info
+        self.n = info.n
+        self.probes = list(range(self.n))
+
+    # -- structure --------------------------------------------------------
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.info.n_params)
+        self.engine.check(self.engine.lib.cs_circuit_default_params(self.engine.ctx, self.cid, L.dptr(p)))
+        return p
+
+    @property
+    def jac_prototype(self):
+        import scipy.sparse as sp
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.info.nnz, np.int32)
+        self.engine.check(self.engine.lib.cs_circuit_pattern(self.engine.ctx, self.cid, L.iptr(rp), L.iptr(ci)))
+        return sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(self.n, self.n))
+
+    def set_probes(self, idx: Sequence[Union[int, str]]):
+        ii = np.array([self.flat.index(i) if isinstance(i, str) else int(i) for i in idx], np.int32)
+        self.engine.check(self.engine.lib.cs_circuit_set_probes(self.engine.ctx, self.cid, len(ii), L.iptr(ii)))
+        self.probes = [int(i) for i in ii]
+        self.info.n_probe = len(ii)
+
+    # -- DAE interface (prob.f.f / prob.f.jac) ---------------------------
+    def f(self, out, du, u, p, t, mode=L.MODE_TRAN):
+        out = np.ascontiguousarray(out, dtype=np.float64) if out is not None else np.zeros(self.n)
+        u = np.ascontiguousarray(u, np.float64)
+        du = None if du is None else np.ascontiguousarray(du, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        self.engine.check(self.engine.lib.cs_residual(self.engine.ctx, self.cid, L.dptr(out),
+                                                      L.dptr(du), L.dptr(u), L.dptr(p), float(t), mode))
+        return out
+
+    def jac(self, vals, du, u, p, gamma, t, mode=L.MODE_TRAN):
+        vals = np.zeros(self.info.nnz) if vals is None else vals
+        u = np.ascontiguousarray(u, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        self.engine.check(self.engine.lib.cs_jacobian(self.engine.ctx, self.cid, L.dptr(vals), None,
+                                                      L.dptr(u), L.dptr(p), float(gamma), float(t), mode))
+        return vals
+
+    def jac_matrix(self, du, u, p, gamma, t, mode=L.MODE_TRAN):
+        import scipy.sparse as sp
+        proto = self.jac_prototype
+        v = self.jac(None, du, u, p, gamma, t, mode)
+        return sp.csr_matrix((v, proto.indices, proto.indptr), shape=proto.shape)
+
+    def lu_refactor(self, vals):
+        vals = np.ascontiguousarray(vals, np.float64)
+        self.engine.check(self.engine.lib.cs_lu_refactor(self.engine.ctx, self.cid, L.dptr(vals)))
+
+    def lu_solve(self, b, transpose=False):
+        x = np.array(b, dtype=np.float64, copy=True)
+        fn = self.engine.lib.cs_lu_solve_transpose if transpose else self.engine.lib.cs_lu_solve
+        self.engine.check(fn(self.engine.ctx, self.cid, L.dptr(x)))
+        return x
+
+    # -- analyses ----------------------------------------------------------
+    def dcop(self, p=None, opts: TranOptions = None) -> "DCSolution":
+        p = self.params() if p is None else np.ascontiguousarray(p, np.float64)
+        o = (opts or TranOptions()).to_c()
+        u = np.zeros(self.n)
+        st = L.cs_tran_stats()
+        rc = self.engine.lib.cs_dcop(self.engine.ctx, self.cid, L.dptr(p), byref(o), L.dptr(u), byref(st))
+        if rc not in (L.CS_OK, L.CS_E_NOCONV):
+            self.engine.check(rc)
+        return DCSolution(self.flat, u, L.RETCODE_NAMES.get(rc, str(rc)), st)
+
+    def tran(self, opts: TranOptions, p=None, u0=None) -> "Solution":
+        p = self.params() if p is None else np.ascontiguousarray(p, np.float64)
+        u0a = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+        o = opts.to_c()
+        npb = len(self.probes)
+        wave = np.zeros((max(opts.n_out, 0), npb))
+        uf = np.zeros(self.n)
+        st = L.cs_tran_stats()
+        rc = self.engine.lib.cs_tran(self.engine.ctx, self.cid, L.dptr(p), L.dptr(u0a), byref(o),
+                                     L.dptr(wave), L.dptr(uf), byref(st))
+        if rc in (L.CS_E_INVALID, L.CS_E_HIP, L.CS_E_OOM, L.CS_E_NODEVICE, L.CS_E_RANGE):
+            self.engine.check(rc)
+        t = opts.t_start + opts.dt_out * np.arange(opts.n_out)
+        return Solution(self.flat, self.probes, t, wave, uf, L.RETCODE_NAMES.get(rc, str(rc)), st)
+
+
+@dataclass
+class DCSolution:
+    flat: FlatCircuit
+    u: np.ndarray
+    retcode: str
+    stats: object
+
+    def __getitem__(self, name):
+        return self.u[self.flat.index(name)] if isinstance(name, str) else self.u[name]
+
+
+class Solution:
+    """Waveforms on the output grid; ``sol["node"]`` / ``sol["V1.I"]``."""
+
+    def __init__(self, flat, probes, t, wave, u_final, retcode, stats):
+        self.flat = flat
+        self.probes = list(probes)
+        self.t = t
+        self.wave = wave
+        self.u_final = u_final
+        self.retcode = retcode
+        self.stats = stats
+        self._col = {p: k for k, p in enumerate(self.probes)}
+
+    def __getitem__(self, name):
+        i = self.flat.index(name) if isinstance(name, str) else int(name)
+        if i not in self._col:
+            raise KeyError(f"{name!r} is not probed")
+        return self.wave[:, self._col[i]]
+
+    def voltage(self, a: str, b: Optional[str] = None):
+        va = self[a]
+        return va - self[b] if b else va
+
+
+# ---------------------------------------------------------------------------
+# entry points mirroring src/sweeps.jl
+# ---------------------------------------------------------------------------
+def dc(circuit, *, engine: Engine = None, p=None, **kw) -> DCSolution:
+    """dc!(circ): DC operating point (src/sweeps.jl:437-448)."""
+    sys = (engine or default_engine()).load(circuit)
+    return sys.dcop(p, TranOptions(**kw) if kw else None)
+
+
+def tran(circuit, tspan, *, engine: Engine = None, p=None, u0=None, n_out=None, dt_out=None,
+         probes=None, **kw) -> Solution:
+    """tran!(circ, tspan; reltol, abstol) (src/sweeps.jl:450-465).  With ``u0``
+    the run starts from that state (UIC) instead of the DC operating point."""
+    sys = (engine or default_engine()).load(circuit)
+    if probes is not None:
+        sys.set_probes(probes)
+    if u0 is not None:
+        kw.setdefault("init", L.INIT_UIC)
+    opts = grid_options(tspan, n_out=n_out, dt_out=dt_out, **kw)
+    return sys.tran(opts, p, u0)
+
+
+class BatchResult:
+    def __init__(self, systems, inst_sys, t, wave, u_final, stats, kernel_ms, traffic):
+        self.systems = systems
+        self.inst_sys = inst_sys
+        self.t = t
+        self.wave = wave          # list of (n_out, n_probe) arrays, one per instance
+        self.u_final = u_final    # list of (n,) arrays
+        self.stats = stats
+        self.kernel_ms = kernel_ms
+        self.alg_bytes, self.alg_flops = traffic
+
+    @property
+    def retcodes(self):
+        return [L.RETCODE_NAMES.get(s.retcode, str(s.retcode)) for s in self.stats]
+
+
+class Batch:
+    """A device-resident batch: create once, run (timed), read results."""
+
+    def __init__(self, engine: Engine, systems: List[CircuitSystem], inst_sys: Sequence[int],
+                 params: Sequence[np.ndarray], opts: TranOptions, u0: Sequence[np.ndarray] = None):
+        self.engine, self.systems, self.opts = engine, systems, opts
+        self.inst_sys = list(inst_sys)
+        cids = np.array([systems[s].cid for s in self.inst_sys], np.int32)
+        self.params = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64) for p in params]))
+        self.u0 = None if u0 is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(x, np.float64) for x in u0]))
+        self._o = opts.to_c()
+        bid = c_int32()
+        engine.check(engine.lib.cs_batch_create(engine.ctx, len(cids), L.iptr(cids), L.dptr(self.params),
+                                                L.dptr(self.u0), byref(self._o), byref(bid)))
+        self.bid = bid.value
+        self.n_inst = len(cids)
+
+    def run(self) -> float:
+        ms = c_double()
+        self.engine.check(self.engine.lib.cs_batch_run(self.engine.ctx, self.bid, byref(ms)))
+        return ms.value
+
+    def results(self, kernel_ms=0.0) -> BatchResult:
+        sysl = [self.systems[s] for s in self.inst_sys]
+        wl = sum(self.opts.n_out * len(s.probes) for s in sysl)
+        ul = sum(s.n for s in sysl)
+        wave = np.zeros(max(wl, 1))
+        uf = np.zeros(ul)
+        st = (L.cs_tran_stats * self.n_inst)()
+        self.engine.check(self.engine.lib.cs_batch_results(self.engine.ctx, self.bid, L.dptr(wave),
+                                                           L.dptr(uf), st))
+        ab, af = c_double(), c_double()
+        self.engine.check(self.engine.lib.cs_batch_traffic(self.engine.ctx, self.bid, byref(ab), byref(af)))
+        waves, ufs = [], []
+        wo = uo = 0
+        for s in sysl:
+            k = self.opts.n_out * len(s.probes)
+            waves.append(wave[wo:wo + k].reshape(self.opts.n_out, len(s.probes)))
+            ufs.append(uf[uo:uo + s.n])
+            wo += k
+            uo += s.n
+        t = self.opts.t_start + self.opts.dt_out * np.arange(self.opts.n_out)
+        return BatchResult(self.systems, self.inst_sys, t, waves, ufs, list(st), kernel_ms,
+                           (ab.value, af.value))
+
+    def close(self):
+        if self.bid is not None:
+            self.engine.lib.cs_batch_destroy(self.engine.ctx, self.bid)
+            self.bid = None
+
+
+def tran_batch(circuits: Sequence, params: Sequence[np.ndarray], opts: TranOptions,
+               u0: Sequence[np.ndarray] = None, engine: Engine = None,
+               probes: Dict[int, Sequence[int]] = None) -> BatchResult:
+    """Batched transient over many instances (the tran!. broadcast of
+    src/sweeps.jl:489-501, as one device launch).  ``circuits`` holds one
+    circuit (or FlatCircuit) per instance; instances whose circuit objects are
+    identical share one compiled system."""
+    eng = engine or default_engine()
+    systems: List[CircuitSystem] = []
+    key_to_sys: Dict[int, int] = {}
+    inst_sys = []
+    for c in circuits:
+        k = id(c)
+        if k not in key_to_sys:
+            key_to_sys[k] = len(systems)
+            systems.append(eng.load(c))
+        inst_sys.append(key_to_sys[k])
+    if probes:
+        for s_idx, pr in probes.items():
+            systems[s_idx].set_probes(pr)
+    b = Batch(eng, systems, inst_sys, params, opts, u0)
+    try:
+        ms = b.run()
+        return b.results(ms)
+    finally:
+        b.close()
+
+
+class CircuitSweep:
+    """Compile a circuit once and run it for many parameter vectors
+    (src/sweeps.jl:390-417)."""
+
+    def __init__(self, circuit, param_sets: Sequence[np.ndarray], engine: Engine = None):
+        self.engine = engine or default_engine()
+        self.system = self.engine.load(circuit)
+        self.param_sets = [np.asarray(p, np.float64) for p in param_sets]
+
+    def __len__(self):
+        return len(self.param_sets)
+
+    def tran(self, opts: TranOptions, u0=None) -> BatchResult:
+        b = Batch(self.engine, [self.system], [0] * len(self), self.param_sets, opts,
+                  None if u0 is None else [u0] * len(self) if np.ndim(u0) == 1 else u0)
+        try:
+            ms = b.run()
+            return b.results(ms)
+        finally:
+            b.close()

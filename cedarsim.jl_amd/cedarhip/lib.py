@@ -1,0 +1,172 @@
+"""ctypes binding of libcedarhip.so (include/cedarhip.h).
+
+This is the Python counterpart of the ``ccall`` stubs in julia/CedarHIP.jl:
+plain pointers and sizes, no torch types.  The library is built in-tree by
+``make -C cedarsim.jl_amd`` (or ``__graft_entry__.build()``); if it is missing
+the import fails loudly instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, byref, c_char_p, c_double, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libcedarhip.so")
+
+CS_OK = 0
+CS_E_INVALID = -1
+CS_E_NODEVICE = -2
+CS_E_HIP = -3
+CS_E_SINGULAR = -4
+CS_E_NOCONV = -5
+CS_E_DTMIN = -6
+CS_E_RANGE = -7
+CS_E_OOM = -8
+CS_E_MAXSTEPS = -9
+
+RETCODE_NAMES = {
+    CS_OK: "Success", CS_E_INVALID: "Invalid", CS_E_NODEVICE: "NoDevice", CS_E_HIP: "HIPError",
+    CS_E_SINGULAR: "Singular", CS_E_NOCONV: "InitialFailure", CS_E_DTMIN: "DtLessThanMin",
+    CS_E_RANGE: "Range", CS_E_OOM: "OutOfMemory", CS_E_MAXSTEPS: "MaxIters",
+}
+
+MODE_TRAN, MODE_DCOP = 0, 1
+INIT_DCOP, INIT_UIC = 0, 1
+
+_I = POINTER(c_int32)
+_D = POINTER(c_double)
+
+
+class cs_netlist(Structure):
+    _fields_ = [
+        ("n_nodes", c_int32),
+        ("n_res", c_int32), ("res_nodes", _I), ("res_r", _D),
+        ("n_cap", c_int32), ("cap_nodes", _I), ("cap_c", _D),
+        ("n_ind", c_int32), ("ind_nodes", _I), ("ind_l", _D),
+        ("n_vsrc", c_int32), ("vsrc_nodes", _I), ("vsrc_wave", _I),
+        ("n_isrc", c_int32), ("isrc_nodes", _I), ("isrc_wave", _I),
+        ("n_vccs", c_int32), ("vccs_nodes", _I), ("vccs_gain", _D),
+        ("n_vcvs", c_int32), ("vcvs_nodes", _I), ("vcvs_gain", _D),
+        ("n_mos", c_int32), ("mos_nodes", _I), ("mos_type", _I),
+        ("mos_vth", _D), ("mos_kp", _D), ("mos_w", _D), ("mos_l", _D),
+        ("n_wave", c_int32), ("wave_kind", _I), ("wave_off", _I), ("wave_len", _I),
+        ("wave_dc", _D),
+        ("n_wave_data", c_int32), ("wave_data", _D),
+        ("gmin", c_double),
+    ]
+
+
+class cs_param_layout(Structure):
+    _fields_ = [(k, c_int32) for k in ("res_r", "cap_c", "ind_l", "vccs_gain", "vcvs_gain",
+                                       "mos_vth", "mos_kp", "mos_w", "mos_l", "wave_dc",
+                                       "wave_data", "gmin", "n_params")]
+
+
+class cs_circuit_info(Structure):
+    _fields_ = [("n", c_int32), ("n_nodes", c_int32), ("nnz", c_int32), ("nnz_lu", c_int32),
+                ("n_params", c_int32), ("lu_levels", c_int32), ("fwd_levels", c_int32),
+                ("bwd_levels", c_int32), ("lu_updates", c_int64), ("n_singletons", c_int32),
+                ("n_probe", c_int32)]
+
+
+class cs_tran_opts(Structure):
+    _fields_ = [("t_start", c_double), ("t_stop", c_double), ("reltol", c_double),
+                ("abstol", c_double), ("dt_init", c_double), ("dt_min", c_double),
+                ("dt_max", c_double), ("newton_tol", c_double), ("dt_out", c_double),
+                ("dc_abstol", c_double), ("n_out", c_int32), ("max_order", c_int32),
+                ("newton_maxit", c_int32), ("init", c_int32), ("max_steps", c_int64),
+                ("lanes", c_int32), ("record_steps", c_int32)]
+
+
+class cs_tran_stats(Structure):
+    _fields_ = [("steps", c_int64), ("rejected", c_int64), ("newton_iters", c_int64),
+                ("t_end", c_double), ("retcode", c_int32), ("pad", c_int32)]
+
+
+# every entry point declared in include/cedarhip.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "cs_abi_version": (c_int32, []),
+    "cs_device_count": (c_int32, [_I]),
+    "cs_create": (c_int32, [c_int32, POINTER(c_void_p)]),
+    "cs_destroy": (c_int32, [c_void_p]),
+    "cs_last_error": (c_char_p, [c_void_p]),
+    "cs_analyze": (c_int32, [POINTER(cs_netlist), POINTER(cs_circuit_info), _I, _I, _I, _I]),
+    "cs_circuit_load": (c_int32, [c_void_p, POINTER(cs_netlist), _I]),
+    "cs_circuit_info_get": (c_int32, [c_void_p, c_int32, POINTER(cs_circuit_info)]),
+    "cs_circuit_layout": (c_int32, [c_void_p, c_int32, POINTER(cs_param_layout)]),
+    "cs_circuit_default_params": (c_int32, [c_void_p, c_int32, _D]),
+    "cs_circuit_pattern": (c_int32, [c_void_p, c_int32, _I, _I]),
+    "cs_circuit_set_probes": (c_int32, [c_void_p, c_int32, c_int32, _I]),
+    "cs_residual": (c_int32, [c_void_p, c_int32, _D, _D, _D, _D, c_double, c_int32]),
+    "cs_jacobian": (c_int32, [c_void_p, c_int32, _D, _D, _D, _D, c_double, c_double, c_int32]),
+    "cs_lu_refactor": (c_int32, [c_void_p, c_int32, _D]),
+    "cs_lu_solve": (c_int32, [c_void_p, c_int32, _D]),
+    "cs_lu_solve_transpose": (c_int32, [c_void_p, c_int32, _D]),
+    "cs_tran_opts_default": (None, [POINTER(cs_tran_opts)]),
+    "cs_dcop": (c_int32, [c_void_p, c_int32, _D, POINTER(cs_tran_opts), _D, POINTER(cs_tran_stats)]),
+    "cs_tran": (c_int32, [c_void_p, c_int32, _D, _D, POINTER(cs_tran_opts), _D, _D,
+                          POINTER(cs_tran_stats)]),
+    "cs_batch_create": (c_int32, [c_void_p, c_int32, _I, _D, _D, POINTER(cs_tran_opts), _I]),
+    "cs_batch_run": (c_int32, [c_void_p, c_int32, _D]),
+    "cs_batch_results": (c_int32, [c_void_p, c_int32, _D, _D, POINTER(cs_tran_stats)]),
+    "cs_batch_destroy": (c_int32, [c_void_p, c_int32]),
+    "cs_batch_traffic": (c_int32, [c_void_p, c_int32, _D, _D]),
+}
+
+_lib = None
+
+
+def load_library(path: str = None) -> ctypes.CDLL:
+    """Load libcedarhip.so and declare every entry point."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"libcedarhip.so not built at {p}; run `make -C cedarsim.jl_amd` "
+                          "or __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_I) if a is not None else None
+
+
+class NetlistStruct:
+    """A cs_netlist pointing into numpy arrays it keeps alive."""
+
+    def __init__(self, flat):
+        a = {k: np.ascontiguousarray(v) for k, v in flat.arrays.items()}
+        self._keep = a
+        s = cs_netlist()
+        s.n_nodes = flat.n_nodes
+        s.n_res, s.res_nodes, s.res_r = len(a["res_r"]), iptr(a["res_nodes"]), dptr(a["res_r"])
+        s.n_cap, s.cap_nodes, s.cap_c = len(a["cap_c"]), iptr(a["cap_nodes"]), dptr(a["cap_c"])
+        s.n_ind, s.ind_nodes, s.ind_l = len(a["ind_l"]), iptr(a["ind_nodes"]), dptr(a["ind_l"])
+        s.n_vsrc, s.vsrc_nodes, s.vsrc_wave = len(a["vsrc_wave"]), iptr(a["vsrc_nodes"]), iptr(a["vsrc_wave"])
+        s.n_isrc, s.isrc_nodes, s.isrc_wave = len(a["isrc_wave"]), iptr(a["isrc_nodes"]), iptr(a["isrc_wave"])
+        s.n_vccs, s.vccs_nodes, s.vccs_gain = len(a["vccs_gain"]), iptr(a["vccs_nodes"]), dptr(a["vccs_gain"])
+        s.n_vcvs, s.vcvs_nodes, s.vcvs_gain = len(a["vcvs_gain"]), iptr(a["vcvs_nodes"]), dptr(a["vcvs_gain"])
+        s.n_mos, s.mos_nodes, s.mos_type = len(a["mos_vth"]), iptr(a["mos_nodes"]), iptr(a["mos_type"])
+        s.mos_vth, s.mos_kp = dptr(a["mos_vth"]), dptr(a["mos_kp"])
+        s.mos_w, s.mos_l = dptr(a["mos_w"]), dptr(a["mos_l"])
+        s.n_wave = len(a["wave_kind"])
+        s.wave_kind, s.wave_off, s.wave_len = iptr(a["wave_kind"]), iptr(a["wave_off"]), iptr(a["wave_len"])
+        s.wave_dc = dptr(a["wave_dc"])
+        s.n_wave_data, s.wave_data = len(a["wave_data"]), dptr(a["wave_data"])
+        s.gmin = flat.gmin
+        self.struct = s

@@ -1,0 +1,458 @@
+"""Circuit description, mirroring CedarSim's Julia circuit DSL.
+
+A circuit is an ordinary function that creates nets and calls devices on them,
+exactly as in the reference (src/simulate_ir.jl:28-140, src/simpledevices.jl):
+
+    def VRcircuit():
+        vcc = Named(net, "vcc")()
+        gnd = Named(net, "gnd")()
+        Named(V(5.0), "V")(vcc, gnd)
+        Named(R(2), "R")(vcc, gnd)
+        Gnd()(gnd)
+
+``lower(VRcircuit)`` runs the function against a recording builder and returns
+a :class:`FlatCircuit`: the flat netlist arrays of include/cedarhip.h plus the
+names of every node and branch current.  The reference instead hands the same
+function to DAECompiler (src/circuitodesystem.jl:147-164); the device set and
+the MNA semantics are the same.
+"""
+from __future__ import annotations
+
+import contextvars
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+
+WAVE_DC, WAVE_PULSE, WAVE_PWL, WAVE_SIN = 0, 1, 2, 3
+
+_builder: contextvars.ContextVar = contextvars.ContextVar("cedarhip_builder", default=None)
+
+
+class CedarError(Exception):
+    """Errors raised while describing or lowering a circuit (src/util.jl:3-21)."""
+
+
+# ---------------------------------------------------------------------------
+# nets
+# ---------------------------------------------------------------------------
+class Net:
+    """A circuit net: one node voltage and one KCL equation (src/simulate_ir.jl:28-50)."""
+
+    __slots__ = ("name", "uid", "ground")
+
+    def __init__(self, name: Optional[str], uid: int):
+        self.name = name
+        self.uid = uid
+        self.ground = False
+
+    def __repr__(self) -> str:
+        return f"Net({self.name!r})"
+
+
+def _current() -> "_Builder":
+    b = _builder.get()
+    if b is None:
+        raise CedarError("circuit elements must be created inside a circuit function "
+                         "being lowered (call lower(fn) or tran(fn, ...))")
+    return b
+
+
+def net(name: Optional[str] = None) -> Net:
+    """Create a net (``net(args...)`` in src/simulate_ir.jl:77)."""
+    return _current().new_net(name)
+
+
+class Named:
+    """``Named(element, name)(nets...)`` (src/simulate_ir.jl:79-92)."""
+
+    def __init__(self, element, name: str):
+        self.element = element
+        self.name = str(name)
+
+    def __call__(self, *nets):
+        if self.element is net:
+            return _current().new_net(self.name)
+        return self.element._instantiate(_current(), nets, self.name)
+
+
+# ---------------------------------------------------------------------------
+# source waveforms (src/spectre_env.jl:144-176)
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class Waveform:
+    kind: int
+    data: tuple
+
+    def value(self, t: float) -> float:
+        """Host-side evaluation, for plotting and checks only."""
+        d = self.data
+        if self.kind == WAVE_DC:
+            return d[0]
+        if self.kind == WAVE_PULSE:
+            v1, v2, td, tr, tf, pw, per = d
+            ts = [td, td + tr, td + tr + pw, td + tr + pw + tf]
+            ys = [v1, v2, v2, v1]
+            return _pwl_at(ts, ys, math.fmod(t, per))
+        if self.kind == WAVE_PWL:
+            return _pwl_at(list(d[0::2]), list(d[1::2]), t)
+        vo, va, fr, td, th, ph, nc = d
+        if td < t < nc / fr:
+            return vo + va * math.exp(-(t - td) * th) * math.sin(
+                math.radians(math.fmod(360.0 * fr * (t - td) + ph, 360.0)))
+        return vo + va * math.sin(math.radians(ph))
+
+
+def _pwl_at(ts, ys, t):
+    i = 1
+    while i <= len(ts) and ts[i - 1] < t:
+        i += 1
+    if i <= len(ts) and ts[i - 1] == t:
+        i += 1
+    if i <= 1:
+        return ys[0]
+    if i > len(ts):
+        return ys[-1]
+    if ys[i - 2] == ys[i - 1]:
+        return ys[i - 1]
+    if ts[i - 1] == ts[i - 2]:
+        return (ys[i - 2] + ys[i - 1]) / 2
+    slope = (ys[i - 1] - ys[i - 2]) / (ts[i - 1] - ts[i - 2])
+    return ys[i - 2] + (t - ts[i - 2]) * slope
+
+
+def pulse(v1, v2, td, tr, tf, pw=math.inf, period=math.inf, count=-1) -> Waveform:
+    """SPICE/Spectre PULSE as the reference defines it: a PWL over one period of
+    ``t mod period`` with corners td, td+tr, td+tr+pw, td+tr+pw+tf
+    (src/spectre_env.jl:153-166)."""
+    return Waveform(WAVE_PULSE, tuple(float(x) for x in (v1, v2, td, tr, tf, pw, period)))
+
+
+def pwl(wave: Sequence[float]) -> Waveform:
+    """``pwl([t0, y0, t1, y1, ...])`` (src/spectre_env.jl:144-151)."""
+    w = [float(x) for x in wave]
+    if len(w) % 2 != 0 or not w:
+        raise CedarError("PWL must have an equal number of x and y values")
+    return Waveform(WAVE_PWL, tuple(w))
+
+
+def spsin(vo, va, freq, td=0.0, theta=0.0, phase=0.0, ncycles=math.inf) -> Waveform:
+    """SPICE SIN source (src/spectre_env.jl:169-176)."""
+    return Waveform(WAVE_SIN, tuple(float(x) for x in (vo, va, freq, td, theta, phase, ncycles)))
+
+
+def _as_wave(v) -> Waveform:
+    if isinstance(v, Waveform):
+        return v
+    return Waveform(WAVE_DC, (float(v),))
+
+
+# ---------------------------------------------------------------------------
+# devices (src/simpledevices.jl)
+# ---------------------------------------------------------------------------
+class _Device:
+    prefix = "X"
+    nterm = 2
+
+    def __call__(self, *nets, dscope=None):
+        return self._instantiate(_current(), nets, dscope)
+
+    def _instantiate(self, b: "_Builder", nets, name):
+        if len(nets) != self.nterm:
+            raise CedarError(f"{type(self).__name__} takes {self.nterm} nets, got {len(nets)}")
+        for n_ in nets:
+            if not isinstance(n_, Net):
+                raise CedarError(f"expected a Net, got {n_!r}")
+        return b.add(self, list(nets), name)
+
+
+class SimpleResistor(_Device):
+    """R: I - V/R (src/simpledevices.jl:49-77; thermal noise term is zero in tran)."""
+    prefix = "R"
+
+    def __init__(self, r=None, *, rsh=50.0, w=1e-6, l=1e-6, narrow=0.0, short=0.0):
+        if r is None:
+            r = rsh * (l - short) / (w - narrow)
+        self.r = float(r)
+        if self.r == 0.0:
+            raise CedarError("resistor with zero resistance; use a voltage source")
+
+
+class SimpleCapacitor(_Device):
+    """C: I - C*dV/dt (src/simpledevices.jl:99-109)."""
+    prefix = "C"
+
+    def __init__(self, c=1.0):
+        self.c = float(c)
+
+
+class SimpleInductor(_Device):
+    """L: V - L*dI/dt (src/simpledevices.jl:122-132)."""
+    prefix = "L"
+
+    def __init__(self, l):
+        self.l = float(l)
+
+
+class VoltageSource(_Device):
+    """V source: V - Vsrc; DC value in :dcop, waveform in :tran
+    (src/simpledevices.jl:274-300)."""
+    prefix = "V"
+
+    def __init__(self, dc=None, tran=None, ac=0.0):
+        if dc is None and tran is None:
+            dc = 0.0
+        self.wave = _as_wave(tran if tran is not None else dc)
+        self.dc = float(dc) if (dc is not None and not isinstance(dc, Waveform)) else math.nan
+
+
+class CurrentSource(_Device):
+    """I source: current Iab flows from A to B through the source
+    (src/simpledevices.jl:315-339)."""
+    prefix = "I"
+
+    def __init__(self, dc=None, tran=None, ac=0.0):
+        if dc is None and tran is None:
+            dc = 0.0
+        self.wave = _as_wave(tran if tran is not None else dc)
+        self.dc = float(dc) if (dc is not None and not isinstance(dc, Waveform)) else math.nan
+
+
+class vcvs(_Device):
+    """E: V(A,B) = gain * V(C,D) (src/simpledevices.jl:341-356)."""
+    prefix = "E"
+    nterm = 4
+
+    def __init__(self, gain=1.0):
+        self.gain = float(gain)
+
+
+class vccs(_Device):
+    """G: I(A->B) = gain * V(C,D) (src/simpledevices.jl:358-373)."""
+    prefix = "G"
+    nterm = 4
+
+    def __init__(self, gain=1.0):
+        self.gain = float(gain)
+
+
+class SquareLawMOSFET(_Device):
+    """Square-law MOSFET, the Verilog-A module ``models/sqlaw.va`` (terminals
+    d, g, s, b).  NMOS ``type=+1``, PMOS ``type=-1`` with a negative vth."""
+    prefix = "M"
+    nterm = 4
+
+    def __init__(self, type=1, vth=0.4, kp=200e-6, w=1e-6, l=0.18e-6):
+        if type not in (1, -1):
+            raise CedarError("type must be +1 (NMOS) or -1 (PMOS)")
+        self.type = int(type)
+        self.vth, self.kp, self.w, self.l = float(vth), float(kp), float(w), float(l)
+
+
+class Gnd:
+    """Marks a net as ground (src/simpledevices.jl:302-313)."""
+
+    def __call__(self, n_: Net, dscope=None):
+        n_.ground = True
+        return None
+
+    def _instantiate(self, b, nets, name):
+        for n_ in nets:
+            n_.ground = True
+
+
+class UnimplementedDevice(_Device):
+    """A device the engine does not model (src/simpledevices.jl:403-411)."""
+
+    def __init__(self, **params):
+        self.params = params
+
+    def _instantiate(self, b, nets, name):
+        raise CedarError("Unimplemented device")
+
+
+# short aliases used throughout the reference's tests (test/common.jl:16-19)
+R = SimpleResistor
+C = SimpleCapacitor
+L = SimpleInductor
+
+
+def V(v):
+    return VoltageSource(dc=v)
+
+
+def I(i):
+    return CurrentSource(dc=i)
+
+
+# ---------------------------------------------------------------------------
+# builder and lowering
+# ---------------------------------------------------------------------------
+@dataclass
+class _Inst:
+    dev: object
+    nets: List[Net]
+    name: str
+
+
+class _Builder:
+    def __init__(self):
+        self.nets: List[Net] = []
+        self.insts: List[_Inst] = []
+        self.counters: Dict[str, int] = {}
+        self.names = set()
+
+    def new_net(self, name):
+        n_ = Net(name, len(self.nets))
+        self.nets.append(n_)
+        return n_
+
+    def add(self, dev, nets, name):
+        if name is None:
+            k = self.counters.get(dev.prefix, 0) + 1
+            self.counters[dev.prefix] = k
+            name = f"{dev.prefix}{k}"
+        if name in self.names:
+            raise CedarError(f"duplicate device name {name!r}")
+        self.names.add(name)
+        self.insts.append(_Inst(dev, nets, name))
+        return None
+
+
+@dataclass
+class FlatCircuit:
+    """The lowered netlist: arrays in the layout of include/cedarhip.h."""
+
+    n_nodes: int
+    arrays: Dict[str, np.ndarray]
+    gmin: float
+    node_names: List[str]
+    unknown_names: List[str]
+    device_names: Dict[str, tuple]       # name -> (kind, index)
+    names: Dict[str, int] = field(default_factory=dict)   # unknown name -> index
+
+    @property
+    def n(self) -> int:
+        return len(self.unknown_names)
+
+    def layout(self) -> Dict[str, int]:
+        a = self.arrays
+        o = 0
+        lay = {}
+        for key, cnt in (("res_r", len(a["res_r"])), ("cap_c", len(a["cap_c"])),
+                         ("ind_l", len(a["ind_l"])), ("vccs_gain", len(a["vccs_gain"])),
+                         ("vcvs_gain", len(a["vcvs_gain"])), ("mos_vth", len(a["mos_vth"])),
+                         ("mos_kp", len(a["mos_vth"])), ("mos_w", len(a["mos_vth"])),
+                         ("mos_l", len(a["mos_vth"])), ("wave_dc", len(a["wave_kind"])),
+                         ("wave_data", len(a["wave_data"])), ("gmin", 1)):
+            lay[key] = o
+            o += cnt
+        lay["n_params"] = o
+        return lay
+
+    def params(self) -> np.ndarray:
+        """The default parameter vector p (include/cedarhip.h layout)."""
+        a = self.arrays
+        return np.concatenate([a["res_r"], a["cap_c"], a["ind_l"], a["vccs_gain"],
+                               a["vcvs_gain"], a["mos_vth"], a["mos_kp"], a["mos_w"],
+                               a["mos_l"], a["wave_dc"], a["wave_data"],
+                               np.array([self.gmin])]).astype(np.float64)
+
+    def index(self, name: str) -> int:
+        """Unknown index of a node name or of a branch current ``"V1.I"``."""
+        try:
+            return self.names[name]
+        except KeyError:
+            raise KeyError(f"no node or branch current named {name!r}") from None
+
+
+def lower(circuit: Callable[[], None], gmin: float = 1e-12) -> FlatCircuit:
+    """Run a circuit function and lower it to the flat netlist."""
+    if isinstance(circuit, FlatCircuit):
+        return circuit
+    b = _Builder()
+    tok = _builder.set(b)
+    try:
+        circuit()
+    finally:
+        _builder.reset(tok)
+    # node numbering: non-ground nets in creation order
+    idx = {}
+    node_names = []
+    for n_ in b.nets:
+        if n_.ground:
+            idx[n_.uid] = -1
+        else:
+            idx[n_.uid] = len(node_names)
+            node_names.append(n_.name if n_.name else f"net{n_.uid}")
+    nn = len(node_names)
+
+    def nodes(inst):
+        return [idx[x.uid] for x in inst.nets]
+
+    lists = {k: [] for k in ("res", "cap", "ind", "vsrc", "isrc", "vccs", "vcvs", "mos")}
+    for inst in b.insts:
+        d = inst.dev
+        key = {SimpleResistor: "res", SimpleCapacitor: "cap", SimpleInductor: "ind",
+               VoltageSource: "vsrc", CurrentSource: "isrc", vccs: "vccs", vcvs: "vcvs",
+               SquareLawMOSFET: "mos"}.get(type(d))
+        if key is None:
+            raise CedarError(f"device type {type(d).__name__} is not supported")
+        lists[key].append(inst)
+
+    waves_kind, waves_off, waves_len, waves_dc, wave_data = [], [], [], [], []
+
+    def add_wave(dev):
+        waves_kind.append(dev.wave.kind)
+        waves_off.append(len(wave_data))
+        waves_len.append(len(dev.wave.data))
+        waves_dc.append(dev.dc)
+        wave_data.extend(dev.wave.data)
+        return len(waves_kind) - 1
+
+    a: Dict[str, np.ndarray] = {}
+    i32 = lambda v, w: np.asarray(v, dtype=np.int32).reshape(-1) if v else np.zeros(0, np.int32)
+    f64 = lambda v: np.asarray(v, dtype=np.float64).reshape(-1)
+    a["res_nodes"] = i32([nodes(x) for x in lists["res"]], 2)
+    a["res_r"] = f64([x.dev.r for x in lists["res"]])
+    a["cap_nodes"] = i32([nodes(x) for x in lists["cap"]], 2)
+    a["cap_c"] = f64([x.dev.c for x in lists["cap"]])
+    a["ind_nodes"] = i32([nodes(x) for x in lists["ind"]], 2)
+    a["ind_l"] = f64([x.dev.l for x in lists["ind"]])
+    a["vsrc_nodes"] = i32([nodes(x) for x in lists["vsrc"]], 2)
+    a["vsrc_wave"] = i32([add_wave(x.dev) for x in lists["vsrc"]], 1)
+    a["isrc_nodes"] = i32([nodes(x) for x in lists["isrc"]], 2)
+    a["isrc_wave"] = i32([add_wave(x.dev) for x in lists["isrc"]], 1)
+    a["vccs_nodes"] = i32([nodes(x) for x in lists["vccs"]], 4)
+    a["vccs_gain"] = f64([x.dev.gain for x in lists["vccs"]])
+    a["vcvs_nodes"] = i32([nodes(x) for x in lists["vcvs"]], 4)
+    a["vcvs_gain"] = f64([x.dev.gain for x in lists["vcvs"]])
+    a["mos_nodes"] = i32([nodes(x) for x in lists["mos"]], 4)
+    a["mos_type"] = i32([x.dev.type for x in lists["mos"]], 1)
+    a["mos_vth"] = f64([x.dev.vth for x in lists["mos"]])
+    a["mos_kp"] = f64([x.dev.kp for x in lists["mos"]])
+    a["mos_w"] = f64([x.dev.w for x in lists["mos"]])
+    a["mos_l"] = f64([x.dev.l for x in lists["mos"]])
+    a["wave_kind"] = i32(waves_kind, 1)
+    a["wave_off"] = i32(waves_off, 1)
+    a["wave_len"] = i32(waves_len, 1)
+    a["wave_dc"] = f64(waves_dc)
+    a["wave_data"] = f64(wave_data)
+
+    unknown_names = list(node_names)
+    names = {nm: i for i, nm in enumerate(node_names)}
+    device_names = {}
+    for k, x in enumerate(lists["vsrc"]):
+        names[f"{x.name}.I"] = len(unknown_names)
+        unknown_names.append(f"{x.name}.I")
+    for k, x in enumerate(lists["ind"]):
+        names[f"{x.name}.I"] = len(unknown_names)
+        unknown_names.append(f"{x.name}.I")
+    for k, x in enumerate(lists["vcvs"]):
+        names[f"{x.name}.I"] = len(unknown_names)
+        unknown_names.append(f"{x.name}.I")
+    for key, lst in lists.items():
+        for k, x in enumerate(lst):
+            device_names[x.name] = (key, k)
+    return FlatCircuit(nn, a, float(gmin), node_names, unknown_names, device_names, names)

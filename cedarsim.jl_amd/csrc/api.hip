@@ -1,0 +1,888 @@
+// api.hip — implementation of the C ABI declared in include/cedarhip.h.
+//
+// Host responsibilities: symbolic analysis (symbolic.cpp), plan upload, the
+// DAE interface (residual / Jacobian / LU on the device), the plain
+// per-instance transient (host step control, one captured hipGraph launch per
+// Newton iteration), and the batched transient (one persistent kernel).
+// There is no CPU compute fallback: without a HIP device every compute entry
+// point returns CS_E_NODEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/cedarhip.h"
+#include "kernels.h"
+#include "plan.h"
+#include "rules.h"
+
+using namespace cedar;
+
+namespace {
+
+struct Circuit {
+    Plan plan;
+    DPlan hd{};                 // device pointers, host copy
+    void* dbuf = nullptr;       // plan arrays
+    DPlan* d_dplan = nullptr;   // device copy of hd
+    // single-instance workspace
+    void* wbuf = nullptr;
+    InstDev w{};
+    double* scratch = nullptr;  // n doubles (charges / du)
+    double* h_norm = nullptr;   // pinned, NRM_SLOTS
+    hipGraphExec_t iter_exec = nullptr;
+    hipGraph_t iter_graph = nullptr;
+};
+
+struct Batch {
+    std::vector<int32_t> cid;
+    std::vector<int64_t> p_off, u_off, w_off;
+    int32_t n_inst = 0;
+    int32_t nr = 1;
+    int32_t nblocks = 0;
+    size_t lds = 0;
+    BatchArgs args{};
+    std::vector<void*> allocs;
+    size_t wave_len = 0, u_len = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+}  // namespace
+
+struct cs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    std::string err;
+    std::vector<std::unique_ptr<Circuit>> circuits;
+    std::vector<std::unique_ptr<Batch>> batches;
+};
+
+namespace {
+
+int fail(cs_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? CS_E_OOM : CS_E_HIP,          \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+Circuit* get_circuit(cs_ctx* ctx, int32_t cid) {
+    if (!ctx || cid < 0 || cid >= (int32_t)ctx->circuits.size() || !ctx->circuits[cid]) return nullptr;
+    return ctx->circuits[cid].get();
+}
+
+// ---- plan upload ------------------------------------------------------------
+struct Packer {
+    std::vector<char> bytes;
+    template <class T>
+    size_t add(const std::vector<T>& v) {
+        size_t off = (bytes.size() + 15) & ~size_t(15);
+        bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 16));
+        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+int upload_plan(cs_ctx* ctx, Circuit& C) {
+    const Plan& P = C.plan;
+    Packer pk;
+    std::vector<int32_t> mos4 = P.mos;
+    std::vector<int4> src4;
+    for (auto& s : P.src) src4.push_back(make_int4(s.kind, s.off, s.len, s.dc));
+    std::vector<int4> task4;
+    for (auto& t : P.lu_task) task4.push_back(make_int4(t.e, t.ub, t.ue, t.piv));
+    std::vector<int2> upd2;
+    for (size_t q = 0; q + 1 < P.lu_upd.size(); q += 2) upd2.push_back(make_int2(P.lu_upd[q], P.lu_upd[q + 1]));
+    size_t o_rowptr = pk.add(P.rowptr), o_colperm = pk.add(P.colperm), o_colnat = pk.add(P.colnat),
+           o_rownat = pk.add(P.rownat), o_pcol = pk.add(P.pcol), o_diag = pk.add(P.diag),
+           o_lk = pk.add(P.lin_kind), o_li = pk.add(P.lin_idx), o_gp = pk.add(P.g_ptr),
+           o_gr = pk.add(P.g_ref), o_cp = pk.add(P.c_ptr), o_cr = pk.add(P.c_ref),
+           o_mos = pk.add(mos4), o_jp = pk.add(P.jnl_ptr), o_jr = pk.add(P.jnl_ref),
+           o_rp = pk.add(P.rnl_ptr), o_rr = pk.add(P.rnl_ref), o_src = pk.add(src4),
+           o_ll = pk.add(P.lu_lptr), o_lt = pk.add(task4), o_lu = pk.add(upd2),
+           o_fl = pk.add(P.f_lptr), o_fr = pk.add(P.f_rows), o_bl = pk.add(P.b_lptr),
+           o_br = pk.add(P.b_rows), o_tcp = pk.add(P.tcol_ptr), o_tcq = pk.add(P.tcol_pos),
+           o_tcr = pk.add(P.tcol_row), o_tfl = pk.add(P.tf_lptr), o_tfr = pk.add(P.tf_rows),
+           o_tbl = pk.add(P.tb_lptr), o_tbr = pk.add(P.tb_rows), o_isd = pk.add(P.is_diff),
+           o_ps = pk.add(P.probe_slot);
+    if (C.dbuf) { (void)hipFree(C.dbuf); C.dbuf = nullptr; }
+    HIPCHK(ctx, hipMalloc(&C.dbuf, pk.bytes.size()));
+    HIPCHK(ctx, hipMemcpy(C.dbuf, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
+    char* b = (char*)C.dbuf;
+    auto I = [&](size_t o) { return (const int32_t*)(b + o); };
+    DPlan& d = C.hd;
+    d.n = P.n; d.n_nodes = P.n_nodes; d.nnz = P.nnz; d.n_mos = P.n_mos; d.n_src = P.n_src;
+    d.n_lin = P.n_lin; d.n_stamp = P.n_stamp;
+    d.lu_nlev = (int32_t)P.lu_lptr.size() - 1; d.f_nlev = (int32_t)P.f_lptr.size() - 1;
+    d.b_nlev = (int32_t)P.b_lptr.size() - 1; d.tf_nlev = (int32_t)P.tf_lptr.size() - 1;
+    d.tb_nlev = (int32_t)P.tb_lptr.size() - 1;
+    d.n_probe = (int32_t)P.probe.size(); d.uic_ok = P.uic_ok ? 1 : 0;
+    d.p_mos_vth = P.lay.mos_vth; d.p_mos_kp = P.lay.mos_kp; d.p_mos_w = P.lay.mos_w;
+    d.p_mos_l = P.lay.mos_l; d.p_gmin = P.lay.gmin; d.n_params = P.lay.n_params;
+    d.rowptr = I(o_rowptr); d.colperm = I(o_colperm); d.colnat = I(o_colnat); d.rownat = I(o_rownat);
+    d.pcol = I(o_pcol); d.diag = I(o_diag);
+    d.lin_kind = I(o_lk); d.lin_idx = I(o_li); d.g_ptr = I(o_gp); d.g_ref = I(o_gr);
+    d.c_ptr = I(o_cp); d.c_ref = I(o_cr); d.mos = I(o_mos);
+    d.jnl_ptr = I(o_jp); d.jnl_ref = I(o_jr); d.rnl_ptr = I(o_rp); d.rnl_ref = I(o_rr);
+    d.src = (const int4*)(b + o_src);
+    d.lu_lptr = I(o_ll); d.lu_task = (const int4*)(b + o_lt); d.lu_upd = (const int2*)(b + o_lu);
+    d.f_lptr = I(o_fl); d.f_rows = I(o_fr); d.b_lptr = I(o_bl); d.b_rows = I(o_br);
+    d.tcol_ptr = I(o_tcp); d.tcol_pos = I(o_tcq); d.tcol_row = I(o_tcr);
+    d.tf_lptr = I(o_tfl); d.tf_rows = I(o_tfr); d.tb_lptr = I(o_tbl); d.tb_rows = I(o_tbr);
+    d.is_diff = I(o_isd); d.probe_slot = I(o_ps);
+    if (!C.d_dplan) HIPCHK(ctx, hipMalloc(&C.d_dplan, sizeof(DPlan)));
+    HIPCHK(ctx, hipMemcpy(C.d_dplan, &C.hd, sizeof(DPlan), hipMemcpyHostToDevice));
+    return CS_OK;
+}
+
+// ---- single-instance workspace ---------------------------------------------------
+int ensure_workspace(cs_ctx* ctx, Circuit& C) {
+    if (C.wbuf) return CS_OK;
+    const Plan& P = C.plan;
+    size_t nd = (size_t)P.lay.n_params + P.n_lin + 3 * (size_t)P.nnz + 6 * (size_t)P.n +
+                P.n_stamp + NRM_SLOTS + 8;
+    size_t bytes = nd * sizeof(double) + sizeof(StepDev) + 64;
+    HIPCHK(ctx, hipMalloc(&C.wbuf, bytes));
+    double* q = (double*)C.wbuf;
+    auto take = [&](size_t k) { double* r = q; q += k; return r; };
+    C.w.p = take(P.lay.n_params);
+    C.w.linbuf = take(P.n_lin);
+    C.w.Glin = take(P.nnz);
+    C.w.Clin = take(P.nnz);
+    C.w.F = take(P.nnz);
+    C.w.inv = take(P.n);
+    C.w.u = take(P.n);
+    C.w.x = take(P.n);
+    C.w.qx = take(P.n);
+    C.w.u0 = take(P.n);
+    C.scratch = take(P.n);
+    C.w.st = take(std::max(P.n_stamp, 1));
+    C.w.norm = take(NRM_SLOTS);
+    C.w.sp = (StepDev*)take(sizeof(StepDev) / sizeof(double) + 1);
+    HIPCHK(ctx, hipMemset(C.wbuf, 0, bytes));
+    HIPCHK(ctx, hipHostMalloc((void**)&C.h_norm, NRM_SLOTS * sizeof(double), hipHostMallocDefault));
+    return CS_OK;
+}
+
+// Graph of one Newton iteration: eval+assemble -> LU -> solves -> update ->
+// copy of the norms to pinned host memory.
+int ensure_graph(cs_ctx* ctx, Circuit& C) {
+    if (C.iter_exec) return CS_OK;
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launch_eval_assemble(C.d_dplan, C.w, ctx->stream);
+    launch_lu(C.d_dplan, C.w, ctx->stream);
+    launch_solve(C.d_dplan, C.w, ctx->stream);
+    launch_update(C.d_dplan, C.w, ctx->stream);
+    (void)hipMemcpyAsync(C.h_norm, C.w.norm, NRM_SLOTS * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    HIPCHK(ctx, hipStreamEndCapture(ctx->stream, &C.iter_graph));
+    HIPCHK(ctx, hipGraphInstantiate(&C.iter_exec, C.iter_graph, nullptr, nullptr, 0));
+    return CS_OK;
+}
+
+int set_step(cs_ctx* ctx, Circuit& C, const StepDev& sp) {
+    HIPCHK(ctx, hipMemcpyAsync(C.w.sp, &sp, sizeof(StepDev), hipMemcpyHostToDevice, ctx->stream));
+    return CS_OK;
+}
+
+int run_iteration(cs_ctx* ctx, Circuit& C) {
+    HIPCHK(ctx, hipGraphLaunch(C.iter_exec, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CS_OK;
+}
+
+int check_params(cs_ctx* ctx, const Plan& P, const double* p) {
+    if (!p) return fail(ctx, CS_E_INVALID, "null parameter vector");
+    return CS_OK;
+}
+
+// Newton for the initial point (DC op or UIC) on the single-instance path.
+int init_point(cs_ctx* ctx, Circuit& C, const double* u0, const cs_tran_opts& o, bool uic,
+               cs_tran_stats* st) {
+    const Plan& P = C.plan;
+    std::vector<double> z(P.n, 0.0);
+    HIPCHK(ctx, hipMemcpyAsync(C.w.u, uic ? u0 : z.data(), sizeof(double) * P.n,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (uic)
+        HIPCHK(ctx, hipMemcpyAsync(C.w.u0, u0, sizeof(double) * P.n, hipMemcpyHostToDevice, ctx->stream));
+    StepDev sp{};
+    sp.a0 = 0.0; sp.t = o.t_start; sp.rtol = o.reltol; sp.atol = o.abstol; sp.tol = o.dc_abstol;
+    sp.mode = uic ? CS_MODE_TRAN : CS_MODE_DCOP;
+    sp.kind = uic ? STEP_UIC : STEP_DC;
+    int rc = set_step(ctx, C, sp);
+    if (rc) return rc;
+    const int maxit = uic ? 50 : 100;
+    for (int it = 0; it < maxit; it++) {
+        if ((rc = run_iteration(ctx, C))) return rc;
+        if (C.h_norm[NRM_DONE] == 1.0) return CS_OK;
+        const double r = C.h_norm[NRM_RES];
+        if (!(r == r) || std::isinf(r)) break;
+        if (st) st->newton_iters++;
+    }
+    return CS_E_NOCONV;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+int cs_abi_version(void) { return CEDARHIP_ABI_VERSION; }
+
+int cs_device_count(int32_t* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (count) *count = (e == hipSuccess) ? n : 0;
+    return (e == hipSuccess && n > 0) ? CS_OK : CS_E_NODEVICE;
+}
+
+int cs_create(int32_t device, cs_ctx** out) {
+    if (!out) return CS_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CS_E_NODEVICE;
+    if (device < 0 || device >= n) return CS_E_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return CS_E_HIP;
+    auto* c = new cs_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CS_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    *out = c;
+    return CS_OK;
+}
+
+int cs_destroy(cs_ctx* ctx) {
+    if (!ctx) return CS_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->batches)
+        if (b) {
+            for (void* p : b->allocs) (void)hipFree(p);
+            if (b->ev0) (void)hipEventDestroy(b->ev0);
+            if (b->ev1) (void)hipEventDestroy(b->ev1);
+        }
+    for (auto& c : ctx->circuits)
+        if (c) {
+            if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
+            if (c->iter_graph) (void)hipGraphDestroy(c->iter_graph);
+            if (c->dbuf) (void)hipFree(c->dbuf);
+            if (c->d_dplan) (void)hipFree(c->d_dplan);
+            if (c->wbuf) (void)hipFree(c->wbuf);
+            if (c->h_norm) (void)hipHostFree(c->h_norm);
+        }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CS_OK;
+}
+
+const char* cs_last_error(const cs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ---- host-only analysis ---------------------------------------------------------------
+static void fill_info(const Plan& P, cs_circuit_info* info) {
+    info->n = P.n;
+    info->n_nodes = P.n_nodes;
+    info->nnz = P.nnz_nat;
+    info->nnz_lu = P.nnz;
+    info->n_params = P.lay.n_params;
+    info->lu_levels = (int32_t)P.lu_lptr.size() - 1;
+    info->fwd_levels = (int32_t)P.f_lptr.size() - 1;
+    info->bwd_levels = (int32_t)P.b_lptr.size() - 1;
+    info->lu_updates = P.lu_updates;
+    info->n_singletons = P.n_singletons;
+    info->n_probe = (int32_t)P.probe.size();
+}
+
+int cs_analyze(const cs_netlist* nl, cs_circuit_info* info, int32_t* lu_rowptr,
+               int32_t* lu_col, int32_t* rownat, int32_t* colnat) {
+    if (!nl || !info) return CS_E_INVALID;
+    Plan P;
+    std::string err;
+    int rc = build_plan(nl, P, err);
+    if (rc != CS_OK) return rc;
+    fill_info(P, info);
+    if (lu_rowptr) std::memcpy(lu_rowptr, P.rowptr.data(), sizeof(int32_t) * (P.n + 1));
+    if (lu_col) std::memcpy(lu_col, P.colperm.data(), sizeof(int32_t) * P.nnz);
+    if (rownat) std::memcpy(rownat, P.rownat.data(), sizeof(int32_t) * P.n);
+    if (colnat)
+        for (int32_t i = 0; i < P.n; i++) colnat[P.pcol[i]] = i;
+    return CS_OK;
+}
+
+// ---- circuits -----------------------------------------------------------------------
+int cs_circuit_load(cs_ctx* ctx, const cs_netlist* nl, int32_t* cid) {
+    if (!ctx || !nl || !cid) return fail(ctx, CS_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    auto C = std::make_unique<Circuit>();
+    std::string err;
+    int rc = build_plan(nl, C->plan, err);
+    if (rc != CS_OK) return fail(ctx, rc, err);
+    if ((rc = upload_plan(ctx, *C)) != CS_OK) return rc;
+    ctx->circuits.push_back(std::move(C));
+    *cid = (int32_t)ctx->circuits.size() - 1;
+    return CS_OK;
+}
+
+int cs_circuit_info_get(cs_ctx* ctx, int32_t cid, cs_circuit_info* info) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !info) return fail(ctx, CS_E_INVALID, "bad circuit id");
+    fill_info(C->plan, info);
+    return CS_OK;
+}
+
+int cs_circuit_layout(cs_ctx* ctx, int32_t cid, cs_param_layout* lay) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !lay) return fail(ctx, CS_E_INVALID, "bad circuit id");
+    *lay = C->plan.lay;
+    return CS_OK;
+}
+
+int cs_circuit_default_params(cs_ctx* ctx, int32_t cid, double* p) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !p) return fail(ctx, CS_E_INVALID, "bad argument");
+    std::memcpy(p, C->plan.p_default.data(), sizeof(double) * C->plan.p_default.size());
+    return CS_OK;
+}
+
+int cs_circuit_pattern(cs_ctx* ctx, int32_t cid, int32_t* rowptr, int32_t* colind) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !rowptr || !colind) return fail(ctx, CS_E_INVALID, "bad argument");
+    std::memcpy(rowptr, C->plan.nat_rowptr.data(), sizeof(int32_t) * (C->plan.n + 1));
+    std::memcpy(colind, C->plan.nat_col.data(), sizeof(int32_t) * C->plan.nnz_nat);
+    return CS_OK;
+}
+
+int cs_circuit_set_probes(cs_ctx* ctx, int32_t cid, int32_t n_probe, const int32_t* idx) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || n_probe < 0 || (n_probe > 0 && !idx)) return fail(ctx, CS_E_INVALID, "bad argument");
+    std::vector<int32_t> v(idx, idx + n_probe);
+    std::vector<char> seen(C->plan.n, 0);
+    for (int32_t i : v) {
+        if (i < 0 || i >= C->plan.n) return fail(ctx, CS_E_INVALID, "probe index out of range");
+        if (seen[i]) return fail(ctx, CS_E_INVALID, "duplicate probe index");
+        seen[i] = 1;
+    }
+    set_probes(C->plan, v);
+    (void)hipSetDevice(ctx->device);
+    return upload_plan(ctx, *C);
+}
+
+// ---- DAE interface ------------------------------------------------------------------
+int cs_residual(cs_ctx* ctx, int32_t cid, double* out, const double* du, const double* u,
+                const double* p, double t, int32_t mode) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !out || !u || !p) return fail(ctx, CS_E_INVALID, "bad argument");
+    (void)hipSetDevice(ctx->device);
+    int rc = ensure_workspace(ctx, *C);
+    if (rc) return rc;
+    const Plan& P = C->plan;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(C->w.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(C->w.u, u, sizeof(double) * P.n, hipMemcpyHostToDevice, s));
+    launch_lin_setup(C->d_dplan, C->w, s);
+    StepDev sp{};
+    sp.t = t;
+    sp.mode = mode;
+    sp.kind = mode == CS_MODE_DCOP ? STEP_DC : STEP_TRAN;
+    sp.a0 = 0.0;
+    if (mode != CS_MODE_DCOP) {
+        std::vector<double> z(P.n, 0.0);
+        HIPCHK(ctx, hipMemcpyAsync(C->scratch, du ? du : z.data(), sizeof(double) * P.n,
+                                   hipMemcpyHostToDevice, s));
+        launch_charges(C->d_dplan, C->w, C->scratch, C->w.qx, s);
+    }
+    if ((rc = set_step(ctx, *C, sp))) return rc;
+    launch_eval_assemble(C->d_dplan, C->w, s);
+    std::vector<double> x(P.n);
+    HIPCHK(ctx, hipMemcpyAsync(x.data(), C->w.x, sizeof(double) * P.n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    for (int k = 0; k < P.n; k++) out[P.rownat[k]] = -x[k];
+    return CS_OK;
+}
+
+int cs_jacobian(cs_ctx* ctx, int32_t cid, double* vals, const double* du, const double* u,
+                const double* p, double gamma, double t, int32_t mode) {
+    (void)du;
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !vals || !u || !p) return fail(ctx, CS_E_INVALID, "bad argument");
+    (void)hipSetDevice(ctx->device);
+    int rc = ensure_workspace(ctx, *C);
+    if (rc) return rc;
+    const Plan& P = C->plan;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(C->w.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(C->w.u, u, sizeof(double) * P.n, hipMemcpyHostToDevice, s));
+    launch_lin_setup(C->d_dplan, C->w, s);
+    StepDev sp{};
+    sp.t = t;
+    sp.mode = mode;
+    sp.kind = mode == CS_MODE_DCOP ? STEP_DC : STEP_TRAN;
+    sp.a0 = gamma;
+    HIPCHK(ctx, hipMemsetAsync(C->w.qx, 0, sizeof(double) * P.n, s));
+    if ((rc = set_step(ctx, *C, sp))) return rc;
+    launch_eval_assemble(C->d_dplan, C->w, s);
+    std::vector<double> F(P.nnz);
+    HIPCHK(ctx, hipMemcpyAsync(F.data(), C->w.F, sizeof(double) * P.nnz, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    for (int q = 0; q < P.nnz_nat; q++) vals[q] = F[P.nat_fpos[q]];
+    return CS_OK;
+}
+
+int cs_lu_refactor(cs_ctx* ctx, int32_t cid, const double* vals) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !vals) return fail(ctx, CS_E_INVALID, "bad argument");
+    (void)hipSetDevice(ctx->device);
+    int rc = ensure_workspace(ctx, *C);
+    if (rc) return rc;
+    const Plan& P = C->plan;
+    std::vector<double> F(P.nnz, 0.0);
+    for (int q = 0; q < P.nnz_nat; q++) F[P.nat_fpos[q]] = vals[q];
+    HIPCHK(ctx, hipMemcpyAsync(C->w.F, F.data(), sizeof(double) * P.nnz, hipMemcpyHostToDevice, ctx->stream));
+    launch_lu(C->d_dplan, C->w, ctx->stream);
+    std::vector<double> inv(P.n);
+    HIPCHK(ctx, hipMemcpyAsync(inv.data(), C->w.inv, sizeof(double) * P.n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < P.n; k++)
+        if (!std::isfinite(inv[k]))
+            return fail(ctx, CS_E_SINGULAR, "zero pivot at step " + std::to_string(k));
+    return CS_OK;
+}
+
+static int lu_solve_common(cs_ctx* ctx, int32_t cid, double* x, bool transpose) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !x) return fail(ctx, CS_E_INVALID, "bad argument");
+    if (!C->wbuf) return fail(ctx, CS_E_INVALID, "cs_lu_refactor must be called first");
+    (void)hipSetDevice(ctx->device);
+    const Plan& P = C->plan;
+    std::vector<double> b(P.n);
+    if (!transpose)
+        for (int k = 0; k < P.n; k++) b[k] = x[P.rownat[k]];
+    else
+        for (int i = 0; i < P.n; i++) b[P.pcol[i]] = x[i];
+    HIPCHK(ctx, hipMemcpyAsync(C->w.x, b.data(), sizeof(double) * P.n, hipMemcpyHostToDevice, ctx->stream));
+    if (!transpose) launch_solve(C->d_dplan, C->w, ctx->stream);
+    else launch_solve_t(C->d_dplan, C->w, ctx->stream);
+    HIPCHK(ctx, hipMemcpyAsync(b.data(), C->w.x, sizeof(double) * P.n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!transpose)
+        for (int i = 0; i < P.n; i++) x[i] = b[P.pcol[i]];
+    else
+        for (int k = 0; k < P.n; k++) x[P.rownat[k]] = b[k];
+    return CS_OK;
+}
+
+int cs_lu_solve(cs_ctx* ctx, int32_t cid, double* x) { return lu_solve_common(ctx, cid, x, false); }
+int cs_lu_solve_transpose(cs_ctx* ctx, int32_t cid, double* x) {
+    return lu_solve_common(ctx, cid, x, true);
+}
+
+// ---- analyses -------------------------------------------------------------------
+void cs_tran_opts_default(cs_tran_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->t_start = 0.0;
+    o->t_stop = 1e-9;
+    o->reltol = 1e-6;
+    o->abstol = 1e-9;
+    o->newton_tol = 1e-2;
+    o->dc_abstol = 1e-10;
+    o->n_out = 0;
+    o->dt_out = 0.0;
+    o->max_order = 2;
+    o->newton_maxit = 8;
+    o->init = CS_INIT_DCOP;
+    o->max_steps = 10000000;
+    o->lanes = 0;
+    o->record_steps = 0;
+}
+
+int cs_dcop(cs_ctx* ctx, int32_t cid, const double* p, const cs_tran_opts* o, double* u_out,
+            cs_tran_stats* st) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !o || !u_out) return fail(ctx, CS_E_INVALID, "bad argument");
+    int rc = check_params(ctx, C->plan, p);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure_workspace(ctx, *C)) || (rc = ensure_graph(ctx, *C))) return rc;
+    const Plan& P = C->plan;
+    HIPCHK(ctx, hipMemcpyAsync(C->w.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, ctx->stream));
+    launch_lin_setup(C->d_dplan, C->w, ctx->stream);
+    cs_tran_stats lst{};
+    if (!st) st = &lst;
+    std::memset(st, 0, sizeof(*st));
+    rc = init_point(ctx, *C, nullptr, *o, false, st);
+    HIPCHK(ctx, hipMemcpy(u_out, C->w.u, sizeof(double) * P.n, hipMemcpyDeviceToHost));
+    st->retcode = rc;
+    st->t_end = o->t_start;
+    if (rc == CS_E_NOCONV) ctx->err = "DC operating point analysis failed";
+    return rc;
+}
+
+// Plain per-instance path: host step control, one graph launch per Newton
+// iteration.  Follows the same rules as the batched kernel (rules.h).
+int cs_tran(cs_ctx* ctx, int32_t cid, const double* p, const double* u0, const cs_tran_opts* o,
+            double* wave, double* u_final, cs_tran_stats* st_out) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !o) return fail(ctx, CS_E_INVALID, "bad argument");
+    int rc = check_params(ctx, C->plan, p);
+    if (rc) return rc;
+    const Plan& P = C->plan;
+    const bool uic = o->init == CS_INIT_UIC;
+    if (uic && !u0) return fail(ctx, CS_E_INVALID, "UIC initialisation needs u0");
+    if (uic && !P.uic_ok)
+        return fail(ctx, CS_E_INVALID, "UIC: a charge-carrying row could not be kept on its own pivot");
+    if (o->n_out > 0 && !(o->dt_out > 0)) return fail(ctx, CS_E_INVALID, "dt_out must be > 0");
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure_workspace(ctx, *C)) || (rc = ensure_graph(ctx, *C))) return rc;
+    hipStream_t s = ctx->stream;
+    const int n = P.n;
+    HIPCHK(ctx, hipMemcpyAsync(C->w.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, s));
+    launch_lin_setup(C->d_dplan, C->w, s);
+    cs_tran_stats st{};
+    rc = init_point(ctx, *C, u0, *o, uic, &st);
+    std::vector<double> U(3 * (size_t)n, 0.0), QH(2 * (size_t)n, 0.0), up(n), u(n), q(n);
+    const StepDefaults sd = step_defaults(*o);
+    double t = o->t_start;
+    const int n_probe = (int)P.probe.size();
+    if (rc == CS_OK) {
+        HIPCHK(ctx, hipMemcpyAsync(U.data(), C->w.u, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        launch_charges(C->d_dplan, C->w, C->w.u, C->scratch, s);
+        HIPCHK(ctx, hipMemcpyAsync(QH.data(), C->scratch, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        double T0 = o->t_start, T1 = 0.0, T2 = 0.0;
+        int npts = 1;
+        double h = sd.dt_init;
+        int kout = 0;
+        if (wave)
+            while (kout < o->n_out && o->t_start + kout * o->dt_out <= o->t_start + 1e-9 * o->dt_out) {
+                for (int s2 = 0; s2 < n_probe; s2++) wave[(size_t)kout * n_probe + s2] = U[P.probe[s2]];
+                kout++;
+            }
+        // host copy of the parameter vector for source breakpoints
+        std::vector<double> ph(p, p + P.lay.n_params);
+        StepDev sp{};
+        sp.rtol = o->reltol; sp.atol = o->abstol; sp.tol = o->dc_abstol;
+        sp.mode = CS_MODE_TRAN; sp.kind = STEP_TRAN;
+        while (t < o->t_stop) {
+            if (st.steps + st.rejected >= o->max_steps) { rc = CS_E_MAXSTEPS; break; }
+            double tb = INFINITY;
+            for (auto& sdsc : P.src) {
+                double b = wave_next_bp(sdsc.kind, ph.data() + sdsc.off, sdsc.len, t);
+                if (b < tb) tb = b;
+            }
+            const bool srcbp = tb < o->t_stop;
+            if (!srcbp) tb = o->t_stop;
+            double hh = h < sd.dt_max ? h : sd.dt_max;
+            int hit;
+            const double tn = clip_step(t, hh, tb, hit);
+            const int k = bdf_order(npts, o->max_order);
+            double a0, a1, a2, w0, w1, w2;
+            bdf_coeffs(k, tn, T0, T1, a0, a1, a2);
+            const int np = npts < k + 1 ? npts : k + 1;
+            pred_weights(np, tn, T0, T1, T2, w0, w1, w2);
+            for (int i = 0; i < n; i++) {
+                double v = U[i];
+                if (np == 2) v = w0 * U[i] + w1 * U[n + i];
+                else if (np == 3) v = w0 * U[i] + w1 * U[n + i] + w2 * U[2 * n + i];
+                up[i] = v;
+                q[i] = a1 * QH[i] + (k == 2 ? a2 * QH[n + i] : 0.0);
+            }
+            sp.a0 = a0;
+            sp.t = tn;
+            HIPCHK(ctx, hipMemcpyAsync(C->w.u, up.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+            HIPCHK(ctx, hipMemcpyAsync(C->w.qx, q.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+            if ((rc = set_step(ctx, *C, sp))) break;
+            bool conv = false;
+            for (int it = 0; it < o->newton_maxit; it++) {
+                if ((rc = run_iteration(ctx, *C))) break;
+                st.newton_iters++;
+                const double nr = C->h_norm[NRM_WRMS];
+                if (!(nr == nr)) break;
+                if (nr <= o->newton_tol) { conv = true; break; }
+            }
+            if (rc) break;
+            if (!conv) {
+                st.rejected++;
+                h = 0.25 * hh;
+                if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
+                continue;
+            }
+            HIPCHK(ctx, hipMemcpyAsync(u.data(), C->w.u, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            launch_charges(C->d_dplan, C->w, C->w.u, C->scratch, s);
+            HIPCHK(ctx, hipMemcpyAsync(q.data(), C->scratch, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipStreamSynchronize(s));
+            double fac = 2.0;
+            if (npts >= 2) {
+                const double c = lte_coeff(k, tn, T0, T1, T2, a0, a1, a2);
+                double ss = 0.0;
+                for (int i = 0; i < n; i++) {
+                    const double est = c * (u[i] - up[i]);
+                    const double m = std::max(std::fabs(u[i]), std::fabs(U[i]));
+                    const double r = est / (o->reltol * m + o->abstol);
+                    ss += r * r;
+                }
+                const double err = std::sqrt(ss / n);
+                if (!(err <= 1.0)) {
+                    st.rejected++;
+                    h = hh * reject_factor(err, k);
+                    if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
+                    continue;
+                }
+                fac = accept_factor(err, k);
+            }
+            std::memmove(U.data() + n, U.data(), sizeof(double) * 2 * n);
+            std::memcpy(U.data(), u.data(), sizeof(double) * n);
+            std::memcpy(QH.data() + n, QH.data(), sizeof(double) * n);
+            std::memcpy(QH.data(), q.data(), sizeof(double) * n);
+            T2 = T1; T1 = T0; T0 = tn;
+            if (npts < 3) npts++;
+            st.steps++;
+            if (wave) {
+                while (kout < o->n_out) {
+                    const double tq = o->t_start + kout * o->dt_out;
+                    if (tq > tn + 1e-9 * o->dt_out) break;
+                    double l0, l1, l2 = 0.0;
+                    if (npts >= 3) {
+                        l0 = (tq - T1) * (tq - T2) / ((T0 - T1) * (T0 - T2));
+                        l1 = (tq - T0) * (tq - T2) / ((T1 - T0) * (T1 - T2));
+                        l2 = (tq - T0) * (tq - T1) / ((T2 - T0) * (T2 - T1));
+                    } else {
+                        const double sl = (tq - T1) / (T0 - T1);
+                        l0 = sl;
+                        l1 = 1.0 - sl;
+                    }
+                    for (int s2 = 0; s2 < n_probe; s2++) {
+                        const int i = P.probe[s2];
+                        wave[(size_t)kout * n_probe + s2] = l0 * U[i] + l1 * U[n + i] + l2 * U[2 * n + i];
+                    }
+                    kout++;
+                }
+            }
+            t = tn;
+            h = hh * fac;
+            if (hit && srcbp) {
+                npts = 1;
+                h = 0.1 * h;
+                if (h < sd.dt_min) h = sd.dt_min;
+            }
+        }
+    }
+    st.t_end = t;
+    st.retcode = rc;
+    if (u_final) {
+        if (st.steps > 0) std::memcpy(u_final, U.data(), sizeof(double) * n);
+        else (void)hipMemcpy(u_final, C->w.u, sizeof(double) * n, hipMemcpyDeviceToHost);
+    }
+    if (st_out) *st_out = st;
+    if (rc == CS_E_DTMIN) ctx->err = "transient: step size fell below dt_min";
+    else if (rc == CS_E_NOCONV) ctx->err = "transient: initial point did not converge";
+    else if (rc == CS_E_MAXSTEPS) ctx->err = "transient: max_steps reached";
+    return rc;
+}
+
+// ---- batched transient --------------------------------------------------------------
+int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const double* params,
+                    const double* u0, const cs_tran_opts* o, int32_t* bid) {
+    if (!ctx || n_inst <= 0 || !inst_cid || !params || !o || !bid)
+        return fail(ctx, CS_E_INVALID, "bad argument");
+    if (o->init == CS_INIT_UIC && !u0) return fail(ctx, CS_E_INVALID, "UIC initialisation needs u0");
+    if (o->n_out > 0 && !(o->dt_out > 0)) return fail(ctx, CS_E_INVALID, "dt_out must be > 0");
+    (void)hipSetDevice(ctx->device);
+    auto B = std::make_unique<Batch>();
+    B->n_inst = n_inst;
+    B->cid.assign(inst_cid, inst_cid + n_inst);
+    int64_t po = 0, uo = 0, wo = 0;
+    int32_t nnz_max = 1, n_max = 1, st_max = 1, lin_max = 1, np_max = 0;
+    std::vector<DPlan> plans(ctx->circuits.size());
+    for (size_t c = 0; c < ctx->circuits.size(); c++)
+        if (ctx->circuits[c]) plans[c] = ctx->circuits[c]->hd;
+    for (int32_t i = 0; i < n_inst; i++) {
+        Circuit* C = get_circuit(ctx, inst_cid[i]);
+        if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id in batch");
+        const Plan& P = C->plan;
+        if (o->init == CS_INIT_UIC && !P.uic_ok)
+            return fail(ctx, CS_E_INVALID, "UIC: a charge-carrying row could not be kept on its own pivot");
+        B->p_off.push_back(po);
+        B->u_off.push_back(uo);
+        B->w_off.push_back(wo);
+        po += P.lay.n_params;
+        uo += P.n;
+        wo += (int64_t)o->n_out * (int64_t)P.probe.size();
+        nnz_max = std::max(nnz_max, P.nnz);
+        n_max = std::max(n_max, P.n);
+        st_max = std::max(st_max, P.n_stamp);
+        lin_max = std::max(lin_max, P.n_lin);
+        np_max = std::max<int32_t>(np_max, (int32_t)P.probe.size());
+    }
+    if (n_max > 64 * 8)
+        return fail(ctx, CS_E_RANGE, "batched path supports up to 512 unknowns per instance; use cs_tran");
+    B->nr = n_max <= 64 ? 1 : n_max <= 128 ? 2 : n_max <= 256 ? 4 : 8;
+    B->u_len = uo;
+    B->wave_len = wo;
+    // largest-first queue order (longest per-iteration work first)
+    std::vector<int32_t> order(n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return ctx->circuits[inst_cid[a]]->plan.nnz > ctx->circuits[inst_cid[b]]->plan.nnz;
+    });
+    auto dalloc = [&](void** p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) return fail(ctx, CS_E_OOM, "batch allocation failed");
+        B->allocs.push_back(*p);
+        return CS_OK;
+    };
+    BatchArgs& A = B->args;
+    int rc;
+    void *d_plans, *d_cid, *d_poff, *d_params, *d_uoff, *d_u0 = nullptr, *d_woff, *d_wave = nullptr,
+                                                    *d_uf, *d_st, *d_order, *d_queue;
+    if ((rc = dalloc(&d_plans, sizeof(DPlan) * plans.size())) ||
+        (rc = dalloc(&d_cid, sizeof(int32_t) * n_inst)) ||
+        (rc = dalloc(&d_poff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_params, sizeof(double) * po)) ||
+        (rc = dalloc(&d_uoff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_woff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_uf, sizeof(double) * uo)) ||
+        (rc = dalloc(&d_st, sizeof(cs_tran_stats) * n_inst)) ||
+        (rc = dalloc(&d_order, sizeof(int32_t) * n_inst)) ||
+        (rc = dalloc(&d_queue, sizeof(int32_t))))
+        return rc;
+    if (u0 && (rc = dalloc(&d_u0, sizeof(double) * uo))) return rc;
+    if (wo > 0 && (rc = dalloc(&d_wave, sizeof(double) * wo))) return rc;
+    HIPCHK(ctx, hipMemcpy(d_plans, plans.data(), sizeof(DPlan) * plans.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_cid, inst_cid, sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_poff, B->p_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_params, params, sizeof(double) * po, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_uoff, B->u_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_woff, B->w_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
+    if (u0) HIPCHK(ctx, hipMemcpy(d_u0, u0, sizeof(double) * uo, hipMemcpyHostToDevice));
+    A.plans = (const DPlan*)d_plans;
+    A.inst_cid = (const int32_t*)d_cid;
+    A.p_off = (const int64_t*)d_poff;
+    A.params = (const double*)d_params;
+    A.u_off = (const int64_t*)d_uoff;
+    A.u0 = (const double*)d_u0;
+    A.w_off = (const int64_t*)d_woff;
+    A.wave = (double*)d_wave;
+    A.u_final = (double*)d_uf;
+    A.stats = (cs_tran_stats*)d_st;
+    A.order = (const int32_t*)d_order;
+    A.queue = (int32_t*)d_queue;
+    A.n_inst = n_inst;
+    A.nnz_max = nnz_max;
+    A.n_max = n_max;
+    A.stamp_max = st_max;
+    A.lin_max = lin_max;
+    A.n_probe_max = np_max;
+    A.o = *o;
+    B->lds = batch_lds_bytes(A);
+    int per_cu = batch_max_blocks(B->nr, B->lds);
+    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "batched kernel does not fit: LDS per instance too large");
+    B->nblocks = std::min<int64_t>(n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+    void *d_g, *d_c, *d_l;
+    if ((rc = dalloc(&d_g, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
+        (rc = dalloc(&d_c, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
+        (rc = dalloc(&d_l, sizeof(double) * (size_t)B->nblocks * lin_max)))
+        return rc;
+    A.glin = (double*)d_g;
+    A.clin = (double*)d_c;
+    A.linbuf = (double*)d_l;
+    HIPCHK(ctx, hipEventCreate(&B->ev0));
+    HIPCHK(ctx, hipEventCreate(&B->ev1));
+    ctx->batches.push_back(std::move(B));
+    *bid = (int32_t)ctx->batches.size() - 1;
+    return CS_OK;
+}
+
+static Batch* get_batch(cs_ctx* ctx, int32_t bid) {
+    if (!ctx || bid < 0 || bid >= (int32_t)ctx->batches.size() || !ctx->batches[bid]) return nullptr;
+    return ctx->batches[bid].get();
+}
+
+int cs_batch_run(cs_ctx* ctx, int32_t bid, double* kernel_ms) {
+    Batch* B = get_batch(ctx, bid);
+    if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemsetAsync(B->args.queue, 0, sizeof(int32_t), s));
+    HIPCHK(ctx, hipEventRecord(B->ev0, s));
+    HIPCHK(ctx, launch_tran_batch(B->nr, B->args, B->nblocks, B->lds, s));
+    HIPCHK(ctx, hipEventRecord(B->ev1, s));
+    HIPCHK(ctx, hipEventSynchronize(B->ev1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, B->ev0, B->ev1));
+    B->last_ms = ms;
+    if (kernel_ms) *kernel_ms = ms;
+    return CS_OK;
+}
+
+int cs_batch_results(cs_ctx* ctx, int32_t bid, double* wave, double* u_final, cs_tran_stats* st) {
+    Batch* B = get_batch(ctx, bid);
+    if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
+    (void)hipSetDevice(ctx->device);
+    if (wave && B->wave_len)
+        HIPCHK(ctx, hipMemcpy(wave, B->args.wave, sizeof(double) * B->wave_len, hipMemcpyDeviceToHost));
+    if (u_final)
+        HIPCHK(ctx, hipMemcpy(u_final, B->args.u_final, sizeof(double) * B->u_len, hipMemcpyDeviceToHost));
+    if (st)
+        HIPCHK(ctx, hipMemcpy(st, B->args.stats, sizeof(cs_tran_stats) * B->n_inst, hipMemcpyDeviceToHost));
+    return CS_OK;
+}
+
+int cs_batch_destroy(cs_ctx* ctx, int32_t bid) {
+    Batch* B = get_batch(ctx, bid);
+    if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : B->allocs) (void)hipFree(p);
+    if (B->ev0) (void)hipEventDestroy(B->ev0);
+    if (B->ev1) (void)hipEventDestroy(B->ev1);
+    ctx->batches[bid].reset();
+    return CS_OK;
+}
+
+// Algorithmic traffic of the executed Newton iterations (per iteration and
+// instance): MOSFET eval 8*(7 in + 4 out) B per device; assembly 8*nnz
+// written + 8*(nnz Glin + nnz Clin + stamp refs) read + 4*(refs + indices);
+// LU 2*8*nnz(L+U); solves 8*nnz(L+U) + 8*3n.  Flops: eval 40 per device,
+// assembly 3 per entry + 1 per stamp ref, LU 2 per update, solves 2*nnz(L+U).
+int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes, double* alg_flops) {
+    Batch* B = get_batch(ctx, bid);
+    if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
+    std::vector<cs_tran_stats> st(B->n_inst);
+    int rc = cs_batch_results(ctx, bid, nullptr, nullptr, st.data());
+    if (rc) return rc;
+    double bytes = 0.0, flops = 0.0;
+    for (int32_t i = 0; i < B->n_inst; i++) {
+        const Plan& P = ctx->circuits[B->cid[i]]->plan;
+        const double refs = (double)P.jnl_ref.size() + (double)P.rnl_ref.size();
+        const double b_it = 8.0 * 11.0 * P.n_mos + 8.0 * P.nnz + 8.0 * (2.0 * P.nnz + refs) +
+                            4.0 * (refs + 2.0 * P.nnz) + 16.0 * P.nnz + 8.0 * P.nnz + 24.0 * P.n;
+        const double f_it = 40.0 * P.n_mos + 3.0 * P.nnz + refs + 2.0 * (double)P.lu_updates +
+                            2.0 * P.nnz;
+        bytes += b_it * (double)st[i].newton_iters;
+        flops += f_it * (double)st[i].newton_iters;
+    }
+    if (alg_bytes) *alg_bytes = bytes;
+    if (alg_flops) *alg_flops = flops;
+    return CS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+// device.h — device building blocks of one Newton iteration, written for a
+// group of NT threads that owns one circuit instance (NT = 64: one wavefront,
+// the batched kernel; NT = 256: the single-instance kernels).
+//
+//   eval       one lane per device instance: square-law MOSFET stamps
+//              (id, dI/dVd, dI/dVg, dI/dVs) and source values into a stamp
+//              buffer in LDS;
+//   assemble   one lane per permuted row: Jacobian entries and residual are
+//              gathered from fixed, ordered contributor lists into the CSR
+//              slots of F (no atomics, bit-reproducible);
+//   LU         entry-level level schedule, one barrier per level;
+//   solves     level-scheduled forward / backward substitution (and the
+//              transposed pair for adjoint solves).
+#pragma once
+#include "dplan.h"
+#include "rules.h"
+
+namespace cedar {
+
+__device__ __forceinline__ double sref_val(const double* buf, int32_t r) {
+    const double v = buf[r >> 1];
+    return (r & 1) ? -v : v;
+}
+
+struct OpSum {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpMax {   // NaN-propagating max
+    __device__ double operator()(double a, double b) const { return (a != a || a > b) ? a : b; }
+};
+struct OpMin {
+    __device__ double operator()(double a, double b) const { return (a != a || a < b) ? a : b; }
+};
+
+// Deterministic block reduction; every thread receives the same value.
+template <int NT, class Op>
+__device__ __forceinline__ double block_reduce(double v, double* red, Op op) {
+    if constexpr (NT == 64) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+        return __shfl(v, 0, 64);
+    } else {
+        red[threadIdx.x] = v;
+        __syncthreads();
+#pragma unroll
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] = op(red[threadIdx.x], red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        const double r = red[0];
+        __syncthreads();
+        return r;
+    }
+}
+
+// Per-instance linear stamp values -> Glin, Clin in F positions.
+template <int NT>
+__device__ void dev_lin_setup(const DPlan* __restrict__ P, const double* __restrict__ p,
+                              double* __restrict__ linbuf, double* __restrict__ Glin,
+                              double* __restrict__ Clin) {
+    const int tid = threadIdx.x;
+    for (int s = tid; s < P->n_lin; s += NT) {
+        const int kd = P->lin_kind[s];
+        const double pv = kd == 0 ? 1.0 : p[P->lin_idx[s]];
+        linbuf[s] = kd == 0 ? 1.0 : kd == 1 ? 1.0 / pv : kd == 2 ? pv : -pv;
+    }
+    __syncthreads();
+    for (int e = tid; e < P->nnz; e += NT) {
+        double g = 0.0, c = 0.0;
+        for (int q = P->g_ptr[e]; q < P->g_ptr[e + 1]; q++) g += sref_val(linbuf, P->g_ref[q]);
+        for (int q = P->c_ptr[e]; q < P->c_ptr[e + 1]; q++) c += sref_val(linbuf, P->c_ref[q]);
+        Glin[e] = g;
+        Clin[e] = c;
+    }
+    __syncthreads();
+}
+
+template <int NT>
+__device__ void dev_sources(const DPlan* __restrict__ P, const double* __restrict__ p,
+                                            double t, int mode, double* __restrict__ st) {
+    for (int s = threadIdx.x; s < P->n_src; s += NT) {
+        const int4 d = P->src[s];
+        st[4 * P->n_mos + s] = source_value(d.x, p + d.y, d.z, p[d.w], t, mode);
+    }
+}
+
+template <int NT>
+__device__ void dev_mos(const DPlan* __restrict__ P, const double* __restrict__ p,
+                                        const double* __restrict__ u, double* __restrict__ st) {
+    const double gmin = p[P->p_gmin];
+    for (int m = threadIdx.x; m < P->n_mos; m += NT) {
+        const int4 dv = reinterpret_cast<const int4*>(P->mos)[m];
+        const double vd = dv.x >= 0 ? u[dv.x] : 0.0;
+        const double vg = dv.y >= 0 ? u[dv.y] : 0.0;
+        const double vs = dv.z >= 0 ? u[dv.z] : 0.0;
+        double id, dd, dg, ds;
+        sqlaw_eval(dv.w, p[P->p_mos_vth + m], p[P->p_mos_kp + m], p[P->p_mos_w + m],
+                   p[P->p_mos_l + m], gmin, vd, vg, vs, id, dd, dg, ds);
+        st[4 * m] = id;
+        st[4 * m + 1] = dd;
+        st[4 * m + 2] = dg;
+        st[4 * m + 3] = ds;
+    }
+}
+
+// F[e] = Glin + a0*Clin + nonlinear stamps;  x[k] = -(residual of row k).
+// qx (natural rows, may be null) is added to the residual.  With pin != null
+// the rows that carry charge become u_r = pin_r (consistent initialisation).
+// Returns this thread's max |x| over its rows.
+template <int NT>
+__device__ double dev_assemble(const DPlan* __restrict__ P, const double* __restrict__ Glin,
+                               const double* __restrict__ Clin, double a0,
+                               const double* __restrict__ u, const double* __restrict__ qx,
+                               const double* __restrict__ st, double* __restrict__ F,
+                               double* __restrict__ x, const double* __restrict__ pin) {
+    double lmax = 0.0;
+    for (int k = threadIdx.x; k < P->n; k += NT) {
+        const int r = P->rownat[k];
+        const int e0 = P->rowptr[k], e1 = P->rowptr[k + 1];
+        double res;
+        if (pin && P->is_diff[r]) {
+            const int dg = P->diag[k];
+            for (int e = e0; e < e1; e++) F[e] = (e == dg) ? 1.0 : 0.0;
+            res = u[r] - pin[r];
+        } else {
+            double acc = 0.0;
+            for (int e = e0; e < e1; e++) {
+                const double jl = Glin[e] + a0 * Clin[e];
+                acc += jl * u[P->colnat[e]];
+                double v = jl;
+                for (int q = P->jnl_ptr[e]; q < P->jnl_ptr[e + 1]; q++) v += sref_val(st, P->jnl_ref[q]);
+                F[e] = v;
+            }
+            if (qx) acc += qx[r];
+            for (int q = P->rnl_ptr[k]; q < P->rnl_ptr[k + 1]; q++) acc += sref_val(st, P->rnl_ref[q]);
+            res = acc;
+        }
+        x[k] = -res;
+        const double a = fabs(res);
+        if (a != a || a > lmax) lmax = a;
+    }
+    return lmax;
+}
+
+// Numeric LU on the fixed pattern, static pivots on the diagonal.
+template <int NT>
+__device__ void dev_lu(const DPlan* __restrict__ P, double* __restrict__ F, double* __restrict__ inv) {
+    for (int l = 0; l < P->lu_nlev; l++) {
+        const int b = P->lu_lptr[l], e = P->lu_lptr[l + 1];
+        for (int q = b + (int)threadIdx.x; q < e; q += NT) {
+            const int4 t = P->lu_task[q];
+            double acc = F[t.x];
+            for (int up = t.y; up < t.z; up++) {
+                const int2 ab = P->lu_upd[up];
+                acc -= F[ab.x] * F[ab.y];
+            }
+            if (t.w >= 0) {
+                F[t.x] = acc * inv[t.w];
+            } else {
+                F[t.x] = acc;
+                if (t.w <= -2) inv[-2 - t.w] = 1.0 / acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// L y = b (unit lower), in place
+template <int NT>
+__device__ void dev_fwd(const DPlan* __restrict__ P, const double* __restrict__ F, double* __restrict__ x) {
+    for (int l = 0; l < P->f_nlev; l++) {
+        for (int q = P->f_lptr[l] + (int)threadIdx.x; q < P->f_lptr[l + 1]; q += NT) {
+            const int k = P->f_rows[q];
+            double s = x[k];
+            for (int e = P->rowptr[k]; e < P->diag[k]; e++) s -= F[e] * x[P->colperm[e]];
+            x[k] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// U x = y, in place
+template <int NT>
+__device__ void dev_bwd(const DPlan* __restrict__ P, const double* __restrict__ F,
+                        const double* __restrict__ inv, double* __restrict__ x) {
+    for (int l = 0; l < P->b_nlev; l++) {
+        for (int q = P->b_lptr[l] + (int)threadIdx.x; q < P->b_lptr[l + 1]; q += NT) {
+            const int k = P->b_rows[q];
+            double s = x[k];
+            for (int e = P->diag[k] + 1; e < P->rowptr[k + 1]; e++) s -= F[e] * x[P->colperm[e]];
+            x[k] = s * inv[k];
+        }
+        __syncthreads();
+    }
+}
+
+// U^T z = b, in place (column access to U)
+template <int NT>
+__device__ void dev_tfwd(const DPlan* __restrict__ P, const double* __restrict__ F,
+                         const double* __restrict__ inv, double* __restrict__ x) {
+    for (int l = 0; l < P->tf_nlev; l++) {
+        for (int q = P->tf_lptr[l] + (int)threadIdx.x; q < P->tf_lptr[l + 1]; q += NT) {
+            const int k = P->tf_rows[q];
+            double s = x[k];
+            for (int c = P->tcol_ptr[k]; c < P->tcol_ptr[k + 1]; c++) {
+                const int i = P->tcol_row[c];
+                if (i < k) s -= F[P->tcol_pos[c]] * x[i];
+            }
+            x[k] = s * inv[k];
+        }
+        __syncthreads();
+    }
+}
+
+// L^T w = z (unit upper), in place (column access to L)
+template <int NT>
+__device__ void dev_tbwd(const DPlan* __restrict__ P, const double* __restrict__ F, double* __restrict__ x) {
+    for (int l = 0; l < P->tb_nlev; l++) {
+        for (int q = P->tb_lptr[l] + (int)threadIdx.x; q < P->tb_lptr[l + 1]; q += NT) {
+            const int k = P->tb_rows[q];
+            double s = x[k];
+            for (int c = P->tcol_ptr[k]; c < P->tcol_ptr[k + 1]; c++) {
+                const int i = P->tcol_row[c];
+                if (i > k) s -= F[P->tcol_pos[c]] * x[i];
+            }
+            x[k] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// q[natural row] = (Clin * v)[row]
+template <int NT>
+__device__ void dev_charges(const DPlan* __restrict__ P, const double* __restrict__ Clin,
+                            const double* __restrict__ v, double* __restrict__ q) {
+    for (int k = threadIdx.x; k < P->n; k += NT) {
+        double acc = 0.0;
+        for (int e = P->rowptr[k]; e < P->rowptr[k + 1]; e++) acc += Clin[e] * v[P->colnat[e]];
+        q[P->rownat[k]] = acc;
+    }
+}
+
+}  // namespace cedar

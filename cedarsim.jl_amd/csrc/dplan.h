@@ -1,0 +1,34 @@
+// dplan.h — device view of a Plan (plan.h): raw pointers into one device
+// allocation per circuit, plus the scalar sizes the kernels branch on.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cedar {
+
+struct DPlan {
+    int32_t n, n_nodes, nnz, n_mos, n_src, n_lin, n_stamp;
+    int32_t lu_nlev, f_nlev, b_nlev, tf_nlev, tb_nlev;
+    int32_t n_probe, uic_ok;
+    int32_t p_mos_vth, p_mos_kp, p_mos_w, p_mos_l, p_gmin, n_params;
+    // F pattern
+    const int32_t *rowptr, *colperm, *colnat, *rownat, *pcol, *diag;
+    // linear stamps
+    const int32_t *lin_kind, *lin_idx, *g_ptr, *g_ref, *c_ptr, *c_ref;
+    // nonlinear devices and gathers
+    const int32_t* mos;        // d, g, s, type per device
+    const int32_t *jnl_ptr, *jnl_ref, *rnl_ptr, *rnl_ref;
+    const int4* src;           // kind, off, len, dc
+    // schedules
+    const int32_t* lu_lptr;
+    const int4* lu_task;       // e, ub, ue, piv
+    const int2* lu_upd;        // (a, b)
+    const int32_t *f_lptr, *f_rows, *b_lptr, *b_rows;
+    const int32_t *tcol_ptr, *tcol_pos, *tcol_row;
+    const int32_t *tf_lptr, *tf_rows, *tb_lptr, *tb_rows;
+    // per natural unknown
+    const int32_t *is_diff, *probe_slot;
+};
+
+}  // namespace cedar

@@ -1,0 +1,64 @@
+// kernels.h — launch interface between the C-ABI layer (api.hip) and the
+// kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/cedarhip.h"
+#include "dplan.h"
+
+namespace cedar {
+
+// Per-iteration parameters of the single-instance path, written by the host.
+struct StepDev {
+    double a0, t, rtol, atol, tol;
+    int32_t mode;   // CS_MODE_*
+    int32_t kind;   // STEP_TRAN, STEP_DC, STEP_UIC
+    int32_t pad[2];
+};
+enum : int32_t { STEP_TRAN = 0, STEP_DC = 1, STEP_UIC = 2 };
+
+// norm[] slots written by the single-instance kernels
+enum : int32_t { NRM_WRMS = 0, NRM_MAXD = 1, NRM_RES = 2, NRM_DONE = 3, NRM_SLOTS = 4 };
+
+// Device workspace of one circuit instance (single-instance path).
+struct InstDev {
+    double *p, *linbuf, *Glin, *Clin, *F, *inv, *u, *x, *qx, *st, *u0, *norm;
+    StepDev* sp;
+};
+
+struct BatchArgs {
+    const DPlan* plans;
+    const int32_t* inst_cid;
+    const int64_t* p_off;
+    const double* params;
+    const int64_t* u_off;
+    const double* u0;        // may be null (DC-op init)
+    const int64_t* w_off;
+    double* wave;            // may be null
+    double* u_final;
+    cs_tran_stats* stats;
+    const int32_t* order;    // work-queue order of instances
+    int32_t* queue;          // work-queue head (zeroed before launch)
+    double *glin, *clin, *linbuf;   // per-workgroup workspace
+    int32_t n_inst, nnz_max, n_max, stamp_max, lin_max, n_probe_max;
+    cs_tran_opts o;
+};
+
+constexpr int kSingleThreads = 256;
+constexpr int kBatchThreads = 64;
+
+// single-instance kernels (one workgroup each)
+void launch_lin_setup(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_eval_assemble(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_lu(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_solve(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_solve_t(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_update(const DPlan* dp, const InstDev& w, hipStream_t s);
+void launch_charges(const DPlan* dp, const InstDev& w, const double* v, double* q, hipStream_t s);
+
+// batched transient: LDS bytes per workgroup and launch
+size_t batch_lds_bytes(const BatchArgs& a);
+int batch_max_blocks(int nr, size_t lds);   // resident workgroups per CU
+hipError_t launch_tran_batch(int nr, const BatchArgs& a, int nblocks, size_t lds, hipStream_t s);
+
+}  // namespace cedar

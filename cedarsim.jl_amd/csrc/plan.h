@@ -1,0 +1,100 @@
+// plan.h — per-topology symbolic plan: what the host computes once per circuit
+// (the role DAECompiler's per-topology compile plays in the reference,
+// src/circuitodesystem.jl:147-164) and what every kernel reads.
+//
+// Index spaces
+//   natural  : MNA unknown order of include/cedarhip.h (nodes, then branch
+//              currents of V sources, inductors, VCVS).
+//   permuted : pivot order k = 0..n-1.  Permuted row k is natural row
+//              rownat[k]; permuted column k is natural unknown colnat_of[k].
+//              The static pivot of step k sits on the diagonal (k,k).
+//   F        : the filled L+U pattern in CSR over permuted rows, columns
+//              sorted ascending; the Jacobian is assembled straight into it.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cedarhip.h"
+
+namespace cedar {
+
+// Encoded signed reference into a value buffer: (index << 1) | negate.
+inline int32_t sref(int32_t idx, bool neg) { return (idx << 1) | (neg ? 1 : 0); }
+
+// Kinds of linear stamp values, computed once per instance from p.
+enum LinKind : int32_t {
+    LIN_ONE = 0,    // 1.0 (incidence of branch currents)
+    LIN_RECIP = 1,  // 1 / p[idx]   (resistor conductance)
+    LIN_PARAM = 2,  // p[idx]       (capacitance, controlled-source gain)
+    LIN_NEG = 3,    // -p[idx]      (inductor: charge -L*I)
+};
+
+// Source evaluated per step: wave kind, data offset (absolute in p), data
+// length, absolute index of the DC value in p.
+struct SrcDesc {
+    int32_t kind, off, len, dc;
+};
+
+// LU task: target F position, update range [ub, ue) in the update list,
+// and piv: -1 plain U entry, -2-k diagonal of row k (also writes
+// inv[k] = 1/U(k,k)), >= 0 L entry scaled by inv[piv].
+struct LuTask {
+    int32_t e, ub, ue, piv;
+};
+
+struct Plan {
+    // sizes
+    int32_t n = 0, n_nodes = 0, nnz_nat = 0, nnz = 0;
+    int32_t n_mos = 0, n_src = 0, n_lin = 0, n_stamp = 0;
+    int32_t n_singletons = 0;
+    cs_param_layout lay{};
+    std::vector<double> p_default;   // the netlist's own parameter values
+
+    // natural Jacobian pattern (CSR) and its F positions
+    std::vector<int32_t> nat_rowptr, nat_col, nat_fpos;
+
+    // F pattern
+    std::vector<int32_t> rowptr, colperm, colnat, rownat, pcol, prow, diag;
+
+    // linear stamps: per linear-buffer slot the value recipe, and per F
+    // position the signed gather lists into that buffer for G and C.
+    std::vector<int32_t> lin_kind, lin_idx;
+    std::vector<int32_t> g_ptr, g_ref, c_ptr, c_ref;
+
+    // nonlinear devices (square-law MOSFET): d, g, s, type per device
+    std::vector<int32_t> mos;
+    // stamp buffer = [4*n_mos MOSFET values][n_src source values]
+    std::vector<int32_t> jnl_ptr, jnl_ref;   // per F position
+    std::vector<int32_t> rnl_ptr, rnl_ref;   // per permuted row
+    std::vector<SrcDesc> src;
+
+    // LU and substitution schedules
+    std::vector<int32_t> lu_lptr;
+    std::vector<LuTask> lu_task;
+    std::vector<int32_t> lu_upd;             // pairs (a, b): F[e] -= F[a]*F[b]
+    std::vector<int32_t> f_lptr, f_rows, b_lptr, b_rows;
+    int64_t lu_updates = 0;
+
+    // transposed substitution (for J^T solves): column access to F
+    std::vector<int32_t> tcol_ptr, tcol_pos, tcol_row;
+    std::vector<int32_t> tf_lptr, tf_rows, tb_lptr, tb_rows;
+
+    // per natural row: carries charge (pinned under UIC)
+    std::vector<int32_t> is_diff;
+    bool uic_ok = true;
+
+    // probes: natural indices recorded in waveforms, and per natural index
+    // its probe slot (-1 if not recorded)
+    std::vector<int32_t> probe, probe_slot;
+};
+
+// Build the plan.  Returns CS_OK or a CS_E_* code with a message in err.
+int build_plan(const cs_netlist* nl, Plan& plan, std::string& err);
+void set_probes(Plan& plan, const std::vector<int32_t>& idx);
+
+// Parameter layout of include/cedarhip.h.
+void param_layout(const cs_netlist* nl, cs_param_layout* lay);
+void default_params(const cs_netlist* nl, double* p);
+
+}  // namespace cedar

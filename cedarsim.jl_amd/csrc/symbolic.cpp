@@ -1,0 +1,596 @@
+// symbolic.cpp — host symbolic analysis, once per circuit topology.
+//
+// Lowers a flat netlist to MNA stamps (the devices of src/simpledevices.jl
+// and the square-law Verilog-A module), then computes everything the kernels
+// need to run Newton iterations without any further host work:
+//   1. the Jacobian pattern and, per pattern entry, ordered contributor lists
+//      (deterministic slot-gather assembly: no atomics, fixed summation order);
+//   2. a static pivot order: singleton peeling (voltage-source rows and
+//      branch-current columns, which cause no fill), a maximum transversal for
+//      a zero-free diagonal, then multiple minimum degree on the symmetrised
+//      remainder (eliminating an independent set of minimum-degree nodes per
+//      round keeps the elimination tree shallow, e.g. cyclic reduction on a
+//      ring oscillator's loop);
+//   3. the exact unsymmetric fill pattern F = L+U (row-merge symbolic LU);
+//   4. an entry-level LU schedule: every F entry is one task
+//      F[e] = (F[e] - sum F[a]*F[b]) [* 1/U(j,j)], grouped in dependency
+//      levels, so a level runs fully in parallel with one barrier after it;
+//   5. level schedules for the forward/backward substitutions and for the
+//      transposed substitutions used by the adjoint.
+// The reference has no counterpart (its Jacobian is dense and IDA factors it
+// densely, src/sweeps.jl:456); this is what makes the GPU path sparse.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+
+#include "plan.h"
+
+namespace cedar {
+
+void param_layout(const cs_netlist* nl, cs_param_layout* L) {
+    int32_t o = 0;
+    L->res_r = o;     o += nl->n_res;
+    L->cap_c = o;     o += nl->n_cap;
+    L->ind_l = o;     o += nl->n_ind;
+    L->vccs_gain = o; o += nl->n_vccs;
+    L->vcvs_gain = o; o += nl->n_vcvs;
+    L->mos_vth = o;   o += nl->n_mos;
+    L->mos_kp = o;    o += nl->n_mos;
+    L->mos_w = o;     o += nl->n_mos;
+    L->mos_l = o;     o += nl->n_mos;
+    L->wave_dc = o;   o += nl->n_wave;
+    L->wave_data = o; o += nl->n_wave_data;
+    L->gmin = o;      o += 1;
+    L->n_params = o;
+}
+
+void default_params(const cs_netlist* nl, double* p) {
+    cs_param_layout L;
+    param_layout(nl, &L);
+    for (int k = 0; k < nl->n_res; k++) p[L.res_r + k] = nl->res_r[k];
+    for (int k = 0; k < nl->n_cap; k++) p[L.cap_c + k] = nl->cap_c[k];
+    for (int k = 0; k < nl->n_ind; k++) p[L.ind_l + k] = nl->ind_l[k];
+    for (int k = 0; k < nl->n_vccs; k++) p[L.vccs_gain + k] = nl->vccs_gain[k];
+    for (int k = 0; k < nl->n_vcvs; k++) p[L.vcvs_gain + k] = nl->vcvs_gain[k];
+    for (int k = 0; k < nl->n_mos; k++) {
+        p[L.mos_vth + k] = nl->mos_vth[k];
+        p[L.mos_kp + k] = nl->mos_kp[k];
+        p[L.mos_w + k] = nl->mos_w[k];
+        p[L.mos_l + k] = nl->mos_l[k];
+    }
+    for (int k = 0; k < nl->n_wave; k++) p[L.wave_dc + k] = nl->wave_dc[k];
+    for (int k = 0; k < nl->n_wave_data; k++) p[L.wave_data + k] = nl->wave_data[k];
+    p[L.gmin] = nl->gmin;
+}
+
+namespace {
+
+struct Contrib {
+    int32_t r, c, ref;   // ref: sref into the linear or stamp buffer
+};
+struct RowContrib {
+    int32_t r, ref;
+};
+
+bool check_node(int32_t v, int32_t n_nodes) { return v >= -1 && v < n_nodes; }
+
+// position of permuted column j inside permuted row i of F, or -1
+inline int32_t fpos(const Plan& P, int32_t i, int32_t j) {
+    auto b = P.colperm.begin() + P.rowptr[i];
+    auto e = P.colperm.begin() + P.rowptr[i + 1];
+    auto it = std::lower_bound(b, e, j);
+    if (it == e || *it != j) return -1;
+    return (int32_t)(it - P.colperm.begin());
+}
+
+// Multiple minimum degree on an undirected graph given as adjacency sets.
+// Returns the elimination order.  Each round takes the minimum degree d,
+// picks (ascending index) a maximal independent set among nodes of degree d,
+// and eliminates them together.
+std::vector<int32_t> mmd_order(std::vector<std::set<int32_t>> adj) {
+    const int32_t m = (int32_t)adj.size();
+    std::vector<int32_t> order;
+    order.reserve(m);
+    std::vector<char> alive(m, 1), mark(m, 0);
+    int32_t left = m;
+    while (left > 0) {
+        size_t dmin = SIZE_MAX;
+        for (int32_t v = 0; v < m; v++)
+            if (alive[v] && adj[v].size() < dmin) dmin = adj[v].size();
+        std::fill(mark.begin(), mark.end(), 0);
+        std::vector<int32_t> pick;
+        for (int32_t v = 0; v < m; v++) {
+            if (!alive[v] || mark[v] || adj[v].size() != dmin) continue;
+            pick.push_back(v);
+            mark[v] = 1;
+            for (int32_t w : adj[v]) mark[w] = 1;
+        }
+        for (int32_t v : pick) {
+            order.push_back(v);
+            alive[v] = 0;
+            left--;
+            std::vector<int32_t> nb(adj[v].begin(), adj[v].end());
+            for (int32_t a : nb) adj[a].erase(v);
+            for (size_t i = 0; i < nb.size(); i++)
+                for (size_t j = i + 1; j < nb.size(); j++) {
+                    adj[nb[i]].insert(nb[j]);
+                    adj[nb[j]].insert(nb[i]);
+                }
+            adj[v].clear();
+        }
+    }
+    return order;
+}
+
+}  // namespace
+
+void set_probes(Plan& P, const std::vector<int32_t>& idx) {
+    P.probe = idx;
+    P.probe_slot.assign(P.n, -1);
+    for (size_t s = 0; s < idx.size(); s++) P.probe_slot[idx[s]] = (int32_t)s;
+}
+
+int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
+    P = Plan();
+    if (!nl || nl->n_nodes < 0) { err = "null netlist"; return CS_E_INVALID; }
+    const int32_t nn = nl->n_nodes;
+    const int32_t bv = nn, bl = bv + nl->n_vsrc, be = bl + nl->n_ind;
+    const int32_t n = be + nl->n_vcvs;
+    if (n <= 0) { err = "circuit has no unknowns"; return CS_E_INVALID; }
+    P.n = n;
+    P.n_nodes = nn;
+    param_layout(nl, &P.lay);
+    const cs_param_layout& L = P.lay;
+    P.p_default.assign(L.n_params, 0.0);
+    default_params(nl, P.p_default.data());
+
+    // ---- 1. stamps ------------------------------------------------------
+    std::vector<Contrib> G, C, J;
+    std::vector<RowContrib> R;
+    auto lin_slot = [&](int32_t kind, int32_t idx) {
+        P.lin_kind.push_back(kind);
+        P.lin_idx.push_back(idx);
+        return (int32_t)P.lin_kind.size() - 1;
+    };
+    lin_slot(LIN_ONE, 0);
+    auto addG = [&](int32_t r, int32_t c, int32_t s, bool neg) {
+        if (r >= 0 && c >= 0) G.push_back({r, c, sref(s, neg)});
+    };
+    auto addC = [&](int32_t r, int32_t c, int32_t s, bool neg) {
+        if (r >= 0 && c >= 0) C.push_back({r, c, sref(s, neg)});
+    };
+    auto addJ = [&](int32_t r, int32_t c, int32_t s, bool neg) {
+        if (r >= 0 && c >= 0) J.push_back({r, c, sref(s, neg)});
+    };
+    auto addR = [&](int32_t r, int32_t s, bool neg) {
+        if (r >= 0) R.push_back({r, sref(s, neg)});
+    };
+    auto bad = [&](const char* what, int32_t k) {
+        err = std::string("terminal out of range in ") + what + " " + std::to_string(k);
+        return CS_E_INVALID;
+    };
+    for (int32_t k = 0; k < nl->n_res; k++) {
+        int32_t a = nl->res_nodes[2 * k], b = nl->res_nodes[2 * k + 1];
+        if (!check_node(a, nn) || !check_node(b, nn)) return bad("resistor", k);
+        int32_t s = lin_slot(LIN_RECIP, L.res_r + k);
+        addG(a, a, s, false); addG(a, b, s, true);
+        addG(b, a, s, true);  addG(b, b, s, false);
+    }
+    for (int32_t k = 0; k < nl->n_cap; k++) {
+        int32_t a = nl->cap_nodes[2 * k], b = nl->cap_nodes[2 * k + 1];
+        if (!check_node(a, nn) || !check_node(b, nn)) return bad("capacitor", k);
+        int32_t s = lin_slot(LIN_PARAM, L.cap_c + k);
+        addC(a, a, s, false); addC(a, b, s, true);
+        addC(b, a, s, true);  addC(b, b, s, false);
+    }
+    for (int32_t k = 0; k < nl->n_ind; k++) {
+        int32_t a = nl->ind_nodes[2 * k], b = nl->ind_nodes[2 * k + 1], j = bl + k;
+        if (!check_node(a, nn) || !check_node(b, nn)) return bad("inductor", k);
+        int32_t s = lin_slot(LIN_NEG, L.ind_l + k);
+        addG(a, j, 0, false); addG(b, j, 0, true);
+        addG(j, a, 0, false); addG(j, b, 0, true);
+        addC(j, j, s, false);
+    }
+    const int32_t nsrc = nl->n_vsrc + nl->n_isrc;
+    const int32_t stamp_src0 = 4 * nl->n_mos;
+    for (int32_t k = 0; k < nl->n_vsrc; k++) {
+        int32_t a = nl->vsrc_nodes[2 * k], b = nl->vsrc_nodes[2 * k + 1], j = bv + k;
+        if (!check_node(a, nn) || !check_node(b, nn)) return bad("voltage source", k);
+        addG(a, j, 0, false); addG(b, j, 0, true);
+        addG(j, a, 0, false); addG(j, b, 0, true);
+        addR(j, stamp_src0 + k, true);
+    }
+    for (int32_t k = 0; k < nl->n_isrc; k++) {
+        int32_t a = nl->isrc_nodes[2 * k], b = nl->isrc_nodes[2 * k + 1];
+        if (!check_node(a, nn) || !check_node(b, nn)) return bad("current source", k);
+        addR(a, stamp_src0 + nl->n_vsrc + k, false);
+        addR(b, stamp_src0 + nl->n_vsrc + k, true);
+    }
+    for (int32_t k = 0; k < nl->n_vccs; k++) {
+        const int32_t* d = nl->vccs_nodes + 4 * k;
+        for (int q = 0; q < 4; q++) if (!check_node(d[q], nn)) return bad("vccs", k);
+        int32_t s = lin_slot(LIN_PARAM, L.vccs_gain + k);
+        addG(d[0], d[2], s, false); addG(d[0], d[3], s, true);
+        addG(d[1], d[2], s, true);  addG(d[1], d[3], s, false);
+    }
+    for (int32_t k = 0; k < nl->n_vcvs; k++) {
+        const int32_t* d = nl->vcvs_nodes + 4 * k;
+        for (int q = 0; q < 4; q++) if (!check_node(d[q], nn)) return bad("vcvs", k);
+        int32_t j = be + k;
+        int32_t s = lin_slot(LIN_PARAM, L.vcvs_gain + k);
+        addG(d[0], j, 0, false); addG(d[1], j, 0, true);
+        addG(j, d[0], 0, false); addG(j, d[1], 0, true);
+        addG(j, d[2], s, true);  addG(j, d[3], s, false);
+    }
+    for (int32_t k = 0; k < nl->n_mos; k++) {
+        const int32_t* d = nl->mos_nodes + 4 * k;
+        for (int q = 0; q < 4; q++) if (!check_node(d[q], nn)) return bad("mosfet", k);
+        int32_t t = nl->mos_type[k];
+        if (t != 1 && t != -1) { err = "mosfet type must be +1 or -1"; return CS_E_INVALID; }
+        int32_t dn = d[0], gn = d[1], sn = d[2];
+        P.mos.insert(P.mos.end(), {dn, gn, sn, t});
+        int32_t s0 = 4 * k;
+        addR(dn, s0, false); addR(sn, s0, true);
+        addJ(dn, dn, s0 + 1, false); addJ(dn, gn, s0 + 2, false); addJ(dn, sn, s0 + 3, false);
+        addJ(sn, dn, s0 + 1, true);  addJ(sn, gn, s0 + 2, true);  addJ(sn, sn, s0 + 3, true);
+    }
+    for (int32_t w = 0; w < nl->n_wave; w++) {
+        int32_t kd = nl->wave_kind[w], ln = nl->wave_len[w], of = nl->wave_off[w];
+        int32_t need = kd == CS_WAVE_DC ? 1 : kd == CS_WAVE_PULSE ? 7 :
+                       kd == CS_WAVE_SIN ? 7 : 2;
+        if (kd < 0 || kd > 3 || ln < need || of < 0 || of + ln > nl->n_wave_data ||
+            (kd == CS_WAVE_PWL && (ln % 2) != 0)) {
+            err = "malformed waveform " + std::to_string(w);
+            return CS_E_INVALID;
+        }
+    }
+    for (int32_t k = 0; k < nsrc; k++) {
+        int32_t w = k < nl->n_vsrc ? nl->vsrc_wave[k] : nl->isrc_wave[k - nl->n_vsrc];
+        if (w < 0 || w >= nl->n_wave) { err = "source references a missing waveform"; return CS_E_INVALID; }
+        P.src.push_back({nl->wave_kind[w], L.wave_data + nl->wave_off[w], nl->wave_len[w],
+                         L.wave_dc + w});
+    }
+    P.n_mos = nl->n_mos;
+    P.n_src = nsrc;
+    P.n_lin = (int32_t)P.lin_kind.size();
+    P.n_stamp = stamp_src0 + nsrc;
+
+    P.is_diff.assign(n, 0);
+    for (auto& c : C) P.is_diff[c.r] = 1;
+
+    // ---- natural pattern -------------------------------------------------
+    std::vector<std::vector<int32_t>> rowc(n);
+    for (auto* v : {&G, &C, &J})
+        for (auto& c : *v) rowc[c.r].push_back(c.c);
+    for (auto& r : rowc) {
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+    }
+    P.nat_rowptr.assign(n + 1, 0);
+    for (int32_t r = 0; r < n; r++) P.nat_rowptr[r + 1] = P.nat_rowptr[r] + (int32_t)rowc[r].size();
+    P.nat_col.reserve(P.nat_rowptr[n]);
+    for (auto& r : rowc) P.nat_col.insert(P.nat_col.end(), r.begin(), r.end());
+    P.nnz_nat = P.nat_rowptr[n];
+    std::vector<std::vector<int32_t>> colr(n);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t c : rowc[r]) colr[c].push_back(r);
+    for (int32_t r = 0; r < n; r++)
+        if (rowc[r].empty()) {
+            err = "structurally singular: unknown " + std::to_string(r) + " has an empty row";
+            return CS_E_SINGULAR;
+        }
+    auto has = [&](int32_t r, int32_t c) {
+        return std::binary_search(rowc[r].begin(), rowc[r].end(), c);
+    };
+
+    // ---- 2. pivot order --------------------------------------------------
+    std::vector<int32_t> prow_of_k, pcol_of_k;    // pivot sequence
+    std::vector<char> ralive(n, 1), calive(n, 1);
+    std::vector<int32_t> rcnt(n), ccnt(n);
+    for (int32_t r = 0; r < n; r++) rcnt[r] = (int32_t)rowc[r].size();
+    for (int32_t c = 0; c < n; c++) ccnt[c] = (int32_t)colr[c].size();
+    auto take = [&](int32_t r, int32_t c) {
+        prow_of_k.push_back(r);
+        pcol_of_k.push_back(c);
+        ralive[r] = 0;
+        calive[c] = 0;
+        for (int32_t cc : rowc[r]) if (calive[cc]) ccnt[cc]--;
+        for (int32_t rr : colr[c]) if (ralive[rr]) rcnt[rr]--;
+    };
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (int32_t r = 0; r < n; r++) {
+            if (!ralive[r] || rcnt[r] != 1) continue;
+            int32_t c = -1;
+            for (int32_t cc : rowc[r]) if (calive[cc]) { c = cc; break; }
+            if (c < 0 || (P.is_diff[r] && c != r)) continue;
+            take(r, c);
+            changed = true;
+        }
+        for (int32_t c = 0; c < n; c++) {
+            if (!calive[c] || ccnt[c] != 1) continue;
+            int32_t r = -1;
+            for (int32_t rr : colr[c]) if (ralive[rr]) { r = rr; break; }
+            if (r < 0 || (P.is_diff[r] && c != r)) continue;
+            take(r, c);
+            changed = true;
+        }
+    }
+    P.n_singletons = (int32_t)prow_of_k.size();
+    for (int32_t r = 0; r < n; r++)
+        if (ralive[r] && rcnt[r] == 0) {
+            err = "structurally singular at unknown " + std::to_string(r);
+            return CS_E_SINGULAR;
+        }
+
+    // maximum transversal on the remaining block (diagonal first, then
+    // augmenting paths)
+    std::vector<int32_t> rem_rows, rem_cols;
+    for (int32_t r = 0; r < n; r++) if (ralive[r]) rem_rows.push_back(r);
+    for (int32_t c = 0; c < n; c++) if (calive[c]) rem_cols.push_back(c);
+    std::vector<int32_t> match_row(n, -1), match_col(n, -1);
+    for (int32_t r : rem_rows)
+        if (calive[r] && has(r, r)) { match_row[r] = r; match_col[r] = r; }
+    std::vector<int32_t> visit(n, -1);
+    for (int32_t r0 : rem_rows) {
+        if (match_row[r0] >= 0) continue;
+        // iterative DFS for an augmenting path from row r0
+        std::vector<std::pair<int32_t, size_t>> stk;
+        std::vector<int32_t> path_col;
+        stk.push_back({r0, 0});
+        bool found = false;
+        while (!stk.empty() && !found) {
+            auto& [r, it] = stk.back();
+            bool pushed = false;
+            while (it < rowc[r].size()) {
+                int32_t c = rowc[r][it++];
+                if (!calive[c] || visit[c] == r0) continue;
+                visit[c] = r0;
+                if (match_col[c] < 0) {
+                    path_col.push_back(c);
+                    found = true;
+                    break;
+                }
+                path_col.push_back(c);
+                stk.push_back({match_col[c], 0});
+                pushed = true;
+                break;
+            }
+            if (found) break;
+            if (!pushed) {
+                stk.pop_back();
+                if (!path_col.empty() && !stk.empty()) path_col.pop_back();
+            }
+        }
+        if (!found) {
+            err = "structurally singular: no transversal covers unknown row " + std::to_string(r0);
+            return CS_E_SINGULAR;
+        }
+        // flip the path: rows on the stack pair with path_col entries
+        for (size_t q = 0; q < stk.size(); q++) {
+            int32_t r = stk[q].first, c = path_col[q];
+            match_col[c] = r;
+            match_row[r] = c;
+        }
+    }
+    // symmetric MMD on the matched remainder
+    const int32_t m = (int32_t)rem_cols.size();
+    std::vector<int32_t> cidx(n, -1);
+    for (int32_t q = 0; q < m; q++) cidx[rem_cols[q]] = q;
+    std::vector<std::set<int32_t>> adj(m);
+    for (int32_t q = 0; q < m; q++) {
+        int32_t r = match_col[rem_cols[q]];
+        for (int32_t c : rowc[r]) {
+            int32_t q2 = cidx[c];
+            if (q2 < 0 || q2 == q) continue;
+            adj[q].insert(q2);
+            adj[q2].insert(q);
+        }
+    }
+    for (int32_t q : mmd_order(std::move(adj))) {
+        int32_t c = rem_cols[q];
+        prow_of_k.push_back(match_col[c]);
+        pcol_of_k.push_back(c);
+    }
+    P.rownat = prow_of_k;
+    std::vector<int32_t> colnat_of_k = pcol_of_k;
+    P.prow.assign(n, -1);
+    P.pcol.assign(n, -1);
+    for (int32_t k = 0; k < n; k++) {
+        P.prow[P.rownat[k]] = k;
+        P.pcol[colnat_of_k[k]] = k;
+    }
+    P.uic_ok = true;
+    for (int32_t r = 0; r < n; r++)
+        if (P.is_diff[r] && P.pcol[r] != P.prow[r]) P.uic_ok = false;
+
+    // ---- 3. symbolic LU (row merge) -----------------------------------------
+    std::vector<std::vector<int32_t>> frow(n), fcolrows(n);
+    for (int32_t k = 0; k < n; k++) {
+        for (int32_t c : rowc[P.rownat[k]]) frow[k].push_back(P.pcol[c]);
+        std::sort(frow[k].begin(), frow[k].end());
+        for (int32_t j : frow[k]) fcolrows[j].push_back(k);
+    }
+    for (int32_t k = 0; k < n; k++) {
+        if (!std::binary_search(frow[k].begin(), frow[k].end(), k)) {
+            err = "internal: zero structural pivot at step " + std::to_string(k);
+            return CS_E_SINGULAR;
+        }
+        std::vector<int32_t> urow;
+        for (int32_t j : frow[k]) if (j > k) urow.push_back(j);
+        if (urow.empty()) continue;
+        for (size_t q = 0; q < fcolrows[k].size(); q++) {
+            int32_t i = fcolrows[k][q];
+            if (i <= k) continue;
+            std::vector<int32_t> merged;
+            merged.reserve(frow[i].size() + urow.size());
+            std::set_union(frow[i].begin(), frow[i].end(), urow.begin(), urow.end(),
+                           std::back_inserter(merged));
+            if (merged.size() != frow[i].size()) {
+                std::vector<int32_t> added;
+                std::set_difference(merged.begin(), merged.end(), frow[i].begin(),
+                                    frow[i].end(), std::back_inserter(added));
+                for (int32_t j : added) fcolrows[j].push_back(i);
+                frow[i].swap(merged);
+            }
+        }
+    }
+    P.rowptr.assign(n + 1, 0);
+    for (int32_t k = 0; k < n; k++) P.rowptr[k + 1] = P.rowptr[k] + (int32_t)frow[k].size();
+    P.nnz = P.rowptr[n];
+    P.colperm.reserve(P.nnz);
+    for (int32_t k = 0; k < n; k++) P.colperm.insert(P.colperm.end(), frow[k].begin(), frow[k].end());
+    P.colnat.resize(P.nnz);
+    for (int32_t e = 0; e < P.nnz; e++) P.colnat[e] = colnat_of_k[P.colperm[e]];
+    P.diag.resize(n);
+    for (int32_t k = 0; k < n; k++) P.diag[k] = fpos(P, k, k);
+
+    P.nat_fpos.resize(P.nnz_nat);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t s = P.nat_rowptr[r]; s < P.nat_rowptr[r + 1]; s++)
+            P.nat_fpos[s] = fpos(P, P.prow[r], P.pcol[P.nat_col[s]]);
+
+    // ---- gather lists ------------------------------------------------------
+    auto build_lists = [&](const std::vector<Contrib>& v, std::vector<int32_t>& ptr,
+                           std::vector<int32_t>& ref) {
+        std::vector<std::vector<int32_t>> per(P.nnz);
+        for (auto& c : v) per[fpos(P, P.prow[c.r], P.pcol[c.c])].push_back(c.ref);
+        ptr.assign(P.nnz + 1, 0);
+        for (int32_t e = 0; e < P.nnz; e++) ptr[e + 1] = ptr[e] + (int32_t)per[e].size();
+        ref.clear();
+        for (auto& l : per) ref.insert(ref.end(), l.begin(), l.end());
+    };
+    build_lists(G, P.g_ptr, P.g_ref);
+    build_lists(C, P.c_ptr, P.c_ref);
+    build_lists(J, P.jnl_ptr, P.jnl_ref);
+    {
+        std::vector<std::vector<int32_t>> per(n);
+        for (auto& rc : R) per[P.prow[rc.r]].push_back(rc.ref);
+        P.rnl_ptr.assign(n + 1, 0);
+        for (int32_t k = 0; k < n; k++) P.rnl_ptr[k + 1] = P.rnl_ptr[k] + (int32_t)per[k].size();
+        for (auto& l : per) P.rnl_ref.insert(P.rnl_ref.end(), l.begin(), l.end());
+    }
+
+    // ---- 4. entry-level LU schedule --------------------------------------------
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> lcol(n);   // (row i, pos) i>k
+    for (int32_t i = 0; i < n; i++)
+        for (int32_t e = P.rowptr[i]; e < P.diag[i]; e++) lcol[P.colperm[e]].push_back({i, e});
+    std::vector<int32_t> lev(P.nnz, -1);
+    struct TaskTmp { int32_t e, piv; std::vector<int32_t> upd; int32_t lev; };
+    std::vector<TaskTmp> tasks;
+    auto gather_upd = [&](int32_t i, int32_t j, int32_t lim, std::vector<int32_t>& upd, int32_t& mx) {
+        for (int32_t e1 = P.rowptr[i]; e1 < P.rowptr[i + 1]; e1++) {
+            int32_t mcol = P.colperm[e1];
+            if (mcol >= lim) break;
+            int32_t e2 = fpos(P, mcol, j);
+            if (e2 < 0) continue;
+            upd.push_back(e1);
+            upd.push_back(e2);
+            mx = std::max(mx, std::max(lev[e1], lev[e2]));
+        }
+    };
+    for (int32_t k = 0; k < n; k++) {
+        for (int32_t e = P.diag[k]; e < P.rowptr[k + 1]; e++) {
+            int32_t j = P.colperm[e];
+            std::vector<int32_t> upd;
+            int32_t mx = -1;
+            gather_upd(k, j, k, upd, mx);
+            if (j == k || !upd.empty()) {
+                lev[e] = mx + 1;
+                tasks.push_back({e, j == k ? -2 - k : -1, std::move(upd), lev[e]});
+            }
+        }
+        for (auto& [i, e] : lcol[k]) {
+            std::vector<int32_t> upd;
+            int32_t mx = lev[P.diag[k]];
+            gather_upd(i, k, k, upd, mx);
+            lev[e] = mx + 1;
+            tasks.push_back({e, k, std::move(upd), lev[e]});
+        }
+    }
+    int32_t nlev = 0;
+    for (auto& t : tasks) nlev = std::max(nlev, t.lev + 1);
+    std::stable_sort(tasks.begin(), tasks.end(), [](const TaskTmp& a, const TaskTmp& b) {
+        return a.lev != b.lev ? a.lev < b.lev : a.e < b.e;
+    });
+    P.lu_lptr.assign(nlev + 1, 0);
+    for (auto& t : tasks) {
+        LuTask lt;
+        lt.e = t.e;
+        lt.piv = t.piv;
+        lt.ub = (int32_t)(P.lu_upd.size() / 2);
+        P.lu_upd.insert(P.lu_upd.end(), t.upd.begin(), t.upd.end());
+        lt.ue = (int32_t)(P.lu_upd.size() / 2);
+        P.lu_task.push_back(lt);
+        P.lu_lptr[t.lev + 1]++;
+    }
+    for (int32_t l = 0; l < nlev; l++) P.lu_lptr[l + 1] += P.lu_lptr[l];
+    P.lu_updates = (int64_t)(P.lu_upd.size() / 2);
+
+    // ---- 5. substitution schedules --------------------------------------------------
+    auto bucket = [](const std::vector<int32_t>& lv, std::vector<int32_t>& lptr,
+                     std::vector<int32_t>& rows) {
+        int32_t nl_ = 0;
+        for (int32_t v : lv) nl_ = std::max(nl_, v + 1);
+        lptr.assign(nl_ + 1, 0);
+        rows.clear();
+        for (int32_t l = 0; l < nl_; l++) {
+            for (int32_t k = 0; k < (int32_t)lv.size(); k++)
+                if (lv[k] == l) rows.push_back(k);
+            lptr[l + 1] = (int32_t)rows.size();
+        }
+    };
+    std::vector<int32_t> lf(n, -1), lb(n, -1);
+    for (int32_t k = 0; k < n; k++) {
+        int32_t mx = -1;
+        bool any = false;
+        for (int32_t e = P.rowptr[k]; e < P.diag[k]; e++) { any = true; mx = std::max(mx, lf[P.colperm[e]]); }
+        lf[k] = any ? mx + 1 : -1;
+    }
+    for (int32_t k = n - 1; k >= 0; k--) {
+        int32_t mx = -1;
+        for (int32_t e = P.diag[k] + 1; e < P.rowptr[k + 1]; e++) mx = std::max(mx, lb[P.colperm[e]]);
+        lb[k] = mx + 1;
+    }
+    bucket(lf, P.f_lptr, P.f_rows);
+    bucket(lb, P.b_lptr, P.b_rows);
+
+    // transposed: column access to F (rows ascending within a column)
+    {
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> cols(n);
+        for (int32_t i = 0; i < n; i++)
+            for (int32_t e = P.rowptr[i]; e < P.rowptr[i + 1]; e++) cols[P.colperm[e]].push_back({i, e});
+        P.tcol_ptr.assign(n + 1, 0);
+        for (int32_t j = 0; j < n; j++) {
+            P.tcol_ptr[j + 1] = P.tcol_ptr[j] + (int32_t)cols[j].size();
+            for (auto& [i, e] : cols[j]) { P.tcol_row.push_back(i); P.tcol_pos.push_back(e); }
+        }
+        // U^T z = b : z[k] needs U(m,k), m<k  -> always a task (scaled by inv)
+        std::vector<int32_t> ltf(n, -1), ltb(n, -1);
+        for (int32_t k = 0; k < n; k++) {
+            int32_t mx = -1;
+            for (int32_t q = P.tcol_ptr[k]; q < P.tcol_ptr[k + 1]; q++)
+                if (P.tcol_row[q] < k) mx = std::max(mx, ltf[P.tcol_row[q]]);
+            ltf[k] = mx + 1;
+        }
+        // L^T w = z : w[k] needs L(i,k), i>k -> task only if any
+        for (int32_t k = n - 1; k >= 0; k--) {
+            int32_t mx = -1;
+            bool any = false;
+            for (int32_t q = P.tcol_ptr[k]; q < P.tcol_ptr[k + 1]; q++)
+                if (P.tcol_row[q] > k) { any = true; mx = std::max(mx, ltb[P.tcol_row[q]]); }
+            ltb[k] = any ? mx + 1 : -1;
+        }
+        bucket(ltf, P.tf_lptr, P.tf_rows);
+        bucket(ltb, P.tb_lptr, P.tb_rows);
+    }
+
+    std::vector<int32_t> all(n);
+    for (int32_t i = 0; i < n; i++) all[i] = i;
+    set_probes(P, all);
+    return CS_OK;
+}
+
+}  // namespace cedar

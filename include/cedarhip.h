@@ -1,0 +1,243 @@
+/*
+ * cedarhip.h — C ABI of the MI355X (gfx950) Newton-transient engine.
+ *
+ * This library replaces, for transient and DC operating-point analysis, the
+ * part of CedarSim's stack that sits below the circuit description:
+ *
+ *   - the DAECompiler-generated residual  f!(out, du, u, p, t)       and
+ *     Jacobian jac!(J, du, u, p, gamma, t), J = dF/du + gamma*dF/d(du)
+ *     (call sites: benchmarks/benchmark_common.jl:138,155; src/ac.jl:82-89),
+ *   - Sundials IDA's BDF / Newton / dense LU loop behind
+ *     tran!(prob) = solve(prob, IDA())            (src/sweeps.jl:450-465),
+ *   - the DC operating point behind dc!(circ) and CedarDCOp
+ *     (src/sweeps.jl:437-448, src/dcop.jl:24-200),
+ *   - the serial sweep broadcast dc!./tran!. over a CircuitSweep
+ *     (src/sweeps.jl:471-501), as one batched launch.
+ *
+ * Conventions (all entry points):
+ *   - return int: CS_OK (0) on success, a negative CS_E_* code on failure;
+ *     cs_last_error() returns a message for the last failure on that context;
+ *   - host buffers are owned by the caller, device buffers by the library;
+ *   - all arithmetic is IEEE FP64;
+ *   - a context is bound to one GPU and one HIP stream and is not re-entrant.
+ *
+ * Unknown ordering ("natural order", used for u, du, residuals, CSR patterns
+ * and waveforms):  node voltages 0..n_nodes-1, then one branch current per
+ * voltage source, per inductor and per VCVS, in that order.  Node index -1 is
+ * ground.  This is standard MNA: the reference's DAE has one current variable
+ * per branch (src/simulate_ir.jl:112-140) which DAECompiler then eliminates for
+ * current-defined branches; the solution in node voltages is the same DAE.
+ *
+ * Residual sign convention: row r of F is the sum of device currents leaving
+ * node r plus d/dt of the charge stored at node r; a voltage-source row is
+ * V(a) - V(b) - E(t); an inductor row is V(a) - V(b) - d/dt(L*I).
+ */
+#ifndef CEDARHIP_H
+#define CEDARHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CEDARHIP_ABI_VERSION 1
+
+/* ---- return codes ---------------------------------------------------- */
+#define CS_OK            0
+#define CS_E_INVALID    -1   /* bad argument / malformed netlist            */
+#define CS_E_NODEVICE   -2   /* no HIP device, or HIP runtime unavailable   */
+#define CS_E_HIP        -3   /* a HIP runtime call failed                   */
+#define CS_E_SINGULAR   -4   /* structurally or numerically singular matrix */
+#define CS_E_NOCONV     -5   /* Newton did not converge (DC op)             */
+#define CS_E_DTMIN      -6   /* transient step fell below dt_min            */
+#define CS_E_RANGE      -7   /* size beyond a documented limit              */
+#define CS_E_OOM        -8   /* device allocation failed                    */
+#define CS_E_MAXSTEPS   -9   /* transient hit max_steps before t_stop       */
+
+/* ---- source waveforms (src/spectre_env.jl:43-69,144-176) -------------- */
+#define CS_WAVE_DC     0  /* data: [v]                                        */
+#define CS_WAVE_PULSE  1  /* data: [v1 v2 td tr tf pw period]  (Inf allowed)  */
+#define CS_WAVE_PWL    2  /* data: [t0 y0 t1 y1 ...] (len = 2*points)         */
+#define CS_WAVE_SIN    3  /* data: [vo va freq td theta phase ncycles]        */
+
+/* ---- analysis modes (src/simulate_ir.jl:24 sim_mode) ------------------- */
+#define CS_MODE_TRAN   0  /* sources follow their waveforms at time t        */
+#define CS_MODE_DCOP   1  /* sources take their DC value; $time == 0         */
+
+/* ---- transient initialisation ---------------------------------------- */
+#define CS_INIT_DCOP   0  /* CedarDCOp: DC operating point at t_start        */
+#define CS_INIT_UIC    1  /* u0 given; unknowns that carry charge are pinned
+                             to u0 and the remaining (algebraic) unknowns are
+                             solved for consistency                          */
+
+/*
+ * Flat netlist: the lowered form of a circuit after netlist import
+ * (make_spectre_circuit, src/spectre.jl:1656-1697).  Node lists are int32
+ * arrays with the per-device terminal count as stride.
+ *
+ *   res   2 terminals (a, b)         SimpleResistor  src/simpledevices.jl:49-77
+ *   cap   2 terminals (a, b)         SimpleCapacitor src/simpledevices.jl:99-109
+ *   ind   2 terminals (a, b)         SimpleInductor  src/simpledevices.jl:122-132
+ *   vsrc  2 terminals (a, b) + wave  VoltageSource   src/simpledevices.jl:274-300
+ *   isrc  2 terminals (a, b) + wave  CurrentSource   src/simpledevices.jl:315-339
+ *   vccs  4 terminals (a, b, c, d)   vccs            src/simpledevices.jl:358-373
+ *   vcvs  4 terminals (a, b, c, d)   vcvs            src/simpledevices.jl:341-356
+ *   mos   4 terminals (d, g, s, b)   square-law MOSFET, a Verilog-A module
+ *         (models/sqlaw.va in this package) loaded the way the reference loads
+ *         any .hdl model (src/spectre.jl:1452-1469, src/ModelLoader.jl:20);
+ *         type +1 = NMOS, -1 = PMOS.
+ */
+typedef struct cs_netlist {
+    int32_t n_nodes;
+    int32_t n_res;  const int32_t* res_nodes;  const double* res_r;
+    int32_t n_cap;  const int32_t* cap_nodes;  const double* cap_c;
+    int32_t n_ind;  const int32_t* ind_nodes;  const double* ind_l;
+    int32_t n_vsrc; const int32_t* vsrc_nodes; const int32_t* vsrc_wave;
+    int32_t n_isrc; const int32_t* isrc_nodes; const int32_t* isrc_wave;
+    int32_t n_vccs; const int32_t* vccs_nodes; const double* vccs_gain;
+    int32_t n_vcvs; const int32_t* vcvs_nodes; const double* vcvs_gain;
+    int32_t n_mos;  const int32_t* mos_nodes;  const int32_t* mos_type;
+    const double* mos_vth; const double* mos_kp; const double* mos_w; const double* mos_l;
+    int32_t n_wave; const int32_t* wave_kind; const int32_t* wave_off; const int32_t* wave_len;
+    const double* wave_dc;     /* per wave: DC-op value; NaN = value at t=0  */
+    int32_t n_wave_data; const double* wave_data;
+    double gmin;               /* SimSpec.gmin (src/simulate_ir.jl:16), 1e-12 */
+} cs_netlist;
+
+/*
+ * Parameter vector p (the DAE's "p", src/circuitodesystem.jl:66-97): every
+ * numeric value of the netlist, so that sweep / Monte Carlo instances share a
+ * topology and differ only in p.  Layout, in this order:
+ *   res_r[n_res] cap_c[n_cap] ind_l[n_ind] vccs_gain[n_vccs] vcvs_gain[n_vcvs]
+ *   mos_vth[n_mos] mos_kp[n_mos] mos_w[n_mos] mos_l[n_mos]
+ *   wave_dc[n_wave] wave_data[n_wave_data] gmin
+ */
+typedef struct cs_param_layout {
+    int32_t res_r, cap_c, ind_l, vccs_gain, vcvs_gain;
+    int32_t mos_vth, mos_kp, mos_w, mos_l;
+    int32_t wave_dc, wave_data, gmin;
+    int32_t n_params;
+} cs_param_layout;
+
+typedef struct cs_circuit_info {
+    int32_t n;            /* unknowns                                   */
+    int32_t n_nodes;
+    int32_t nnz;          /* entries of the Jacobian pattern            */
+    int32_t nnz_lu;       /* entries of L+U after fill                  */
+    int32_t n_params;
+    int32_t lu_levels;    /* levels of the entry-level LU schedule      */
+    int32_t fwd_levels;   /* levels of the forward substitution         */
+    int32_t bwd_levels;   /* levels of the backward substitution        */
+    int64_t lu_updates;   /* multiply-subtract pairs per refactorization */
+    int32_t n_singletons; /* pivots taken by singleton peeling          */
+    int32_t n_probe;      /* recorded unknowns per output sample        */
+} cs_circuit_info;
+
+typedef struct cs_tran_opts {
+    double  t_start, t_stop;
+    double  reltol, abstol;      /* LTE weights: reltol*|u| + abstol          */
+    double  dt_init;             /* first step after a (re)start; 0 = auto    */
+    double  dt_min, dt_max;      /* 0 = auto                                  */
+    double  newton_tol;          /* WRMS of the Newton update, tolerance units */
+    double  dt_out;              /* output grid spacing                       */
+    double  dc_abstol;           /* DC-op residual tolerance (CedarDCOp 1e-10)*/
+    int32_t n_out;               /* output samples at t_start + k*dt_out      */
+    int32_t max_order;           /* BDF order cap: 1 or 2                     */
+    int32_t newton_maxit;
+    int32_t init;                /* CS_INIT_DCOP or CS_INIT_UIC               */
+    int64_t max_steps;
+    int32_t lanes;               /* batched: threads per instance, 0 = tuned  */
+    int32_t record_steps;        /* keep the accepted-step sequence (adjoint) */
+} cs_tran_opts;
+
+typedef struct cs_tran_stats {
+    int64_t steps;               /* accepted steps                     */
+    int64_t rejected;            /* LTE or Newton rejections           */
+    int64_t newton_iters;        /* Newton iterations = refactorizations */
+    double  t_end;               /* time reached                       */
+    int32_t retcode;             /* CS_OK or CS_E_*                    */
+    int32_t pad;
+} cs_tran_stats;
+
+typedef struct cs_ctx cs_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+int         cs_abi_version(void);
+int         cs_device_count(int32_t* count);
+int         cs_create(int32_t device, cs_ctx** out);
+int         cs_destroy(cs_ctx* ctx);
+const char* cs_last_error(const cs_ctx* ctx);
+
+/* ---- host-only symbolic analysis (no device needed) -------------------- */
+/* Runs the same analysis as cs_circuit_load and reports it: sizes in info;
+ * when the arrays are non-NULL also the filled L+U pattern over permuted rows
+ * (lu_rowptr[n+1], lu_col[nnz_lu], permuted column indices) and the pivot
+ * order (rownat[k] / colnat[k]: natural row / unknown of pivot k).  Lets a
+ * caller inspect the static pivoting and fill without a GPU. */
+int cs_analyze(const cs_netlist* nl, cs_circuit_info* info, int32_t* lu_rowptr,
+               int32_t* lu_col, int32_t* rownat, int32_t* colnat);
+
+/* ---- circuits (host symbolic analysis, done once per topology) --------- */
+/* Lowers the netlist, builds the Jacobian pattern, the slot maps for
+ * deterministic assembly, the static pivot order and the LU / substitution
+ * level schedules, and uploads them.  Replaces DAECompiler's per-topology
+ * compile (src/circuitodesystem.jl:147-164). */
+int cs_circuit_load(cs_ctx* ctx, const cs_netlist* nl, int32_t* cid);
+int cs_circuit_info_get(cs_ctx* ctx, int32_t cid, cs_circuit_info* info);
+int cs_circuit_layout(cs_ctx* ctx, int32_t cid, cs_param_layout* layout);
+int cs_circuit_default_params(cs_ctx* ctx, int32_t cid, double* p);
+/* CSR pattern of J in the natural order: rowptr[n+1], colind[nnz]
+ * (the reference's jac_prototype would be this sparsity). */
+int cs_circuit_pattern(cs_ctx* ctx, int32_t cid, int32_t* rowptr, int32_t* colind);
+/* Unknowns recorded in waveforms (default: all n). */
+int cs_circuit_set_probes(cs_ctx* ctx, int32_t cid, int32_t n_probe, const int32_t* idx);
+
+/* ---- DAE interface, mirrors f!/jac! (benchmarks/benchmark_common.jl:138,155)
+ * F(du,u,p,t) = i(u,p,t) + dQ/du(u,p) * du;  J = di/du + gamma * dQ/du.
+ * out[n]; vals[nnz] in the CSR order of cs_circuit_pattern.               */
+int cs_residual(cs_ctx* ctx, int32_t cid, double* out, const double* du,
+                const double* u, const double* p, double t, int32_t mode);
+int cs_jacobian(cs_ctx* ctx, int32_t cid, double* vals, const double* du,
+                const double* u, const double* p, double gamma, double t,
+                int32_t mode);
+/* Sparse LU on the fixed symbolic pattern (replaces IDA's dense LU). */
+int cs_lu_refactor(cs_ctx* ctx, int32_t cid, const double* vals);
+int cs_lu_solve(cs_ctx* ctx, int32_t cid, double* x);            /* J x = b   */
+int cs_lu_solve_transpose(cs_ctx* ctx, int32_t cid, double* x);  /* J^T x = b */
+
+/* ---- analyses ---------------------------------------------------------- */
+void cs_tran_opts_default(cs_tran_opts* o);
+/* dc!(circ): Newton on F(0,u,p,0)=0 in DC mode (src/dcop.jl:53-155). */
+int cs_dcop(cs_ctx* ctx, int32_t cid, const double* p, const cs_tran_opts* o,
+            double* u_out, cs_tran_stats* st);
+/* tran!(circ, tspan): the plain per-instance GPU path.  The host runs the
+ * BDF step control; every Newton iteration is one launch of a captured
+ * hipGraph (device eval -> assembly -> LU refactor -> solve -> norm).
+ * wave: n_out x n_probe samples; u_final: n.  u0 may be NULL for INIT_DCOP. */
+int cs_tran(cs_ctx* ctx, int32_t cid, const double* p, const double* u0,
+            const cs_tran_opts* o, double* wave, double* u_final,
+            cs_tran_stats* st);
+
+/* ---- batched transient (sweep / Monte Carlo in one launch) ------------- *
+ * Instances may use different circuits (uneven sizes).  params holds each
+ * instance's parameter vector back to back; u0 (may be NULL) each instance's
+ * initial state back to back.  cs_batch_create uploads everything; run
+ * launches one persistent kernel that walks a work queue of instances, each
+ * doing its whole transient on the device; results downloads.             */
+int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid,
+                    const double* params, const double* u0,
+                    const cs_tran_opts* o, int32_t* bid);
+int cs_batch_run(cs_ctx* ctx, int32_t bid, double* kernel_ms);
+int cs_batch_results(cs_ctx* ctx, int32_t bid, double* wave, double* u_final,
+                     cs_tran_stats* st);
+int cs_batch_destroy(cs_ctx* ctx, int32_t bid);
+/* Roofline bookkeeping of the last run: algorithmic bytes moved by the
+ * executed Newton iterations (device eval + assembly + LU + solves). */
+int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes,
+                     double* alg_flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CEDARHIP_H */

@@ -1,0 +1,67 @@
+/*
+ * oracle.h — CPU FP64 restatement of CedarSim's Newton-transient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP library in
+ * cedarsim.jl_amd/ and the timed CPU baseline of bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product never links, imports or calls it.
+ *
+ * The netlist and parameter-vector formats are the documented ones of
+ * include/cedarhip.h (a data format, shared on purpose so both sides read the
+ * same inputs); every algorithm below is written independently of the HIP
+ * library: dense MNA matrices, LAPACK-style partial-pivot LU, plain loops.
+ */
+#ifndef CEDAR_ORACLE_H
+#define CEDAR_ORACLE_H
+#include "../include/cedarhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameter layout of include/cedarhip.h, computed independently. */
+void   or_layout(const cs_netlist* nl, cs_param_layout* lay);
+void   or_default_params(const cs_netlist* nl, double* p);
+int    or_num_unknowns(const cs_netlist* nl);
+
+/* Source waveform value and next breakpoint (src/spectre_env.jl:15-176). */
+double or_wave_value(int kind, const double* d, int len, double t);
+double or_wave_next_bp(int kind, const double* d, int len, double t);
+double or_pwl_at_time(const double* tsys, int npts, double t);
+
+/* Square-law MOSFET: current d->s and its partials wrt Vd, Vg, Vs. */
+void   or_sqlaw(int type, double vth, double kp, double w, double l, double gmin,
+                double vd, double vg, double vs,
+                double* id, double* dvd, double* dvg, double* dvs);
+
+/* Residual / Jacobian of the DAE (mirror of f!/jac!): dense row-major J. */
+int    or_residual(const cs_netlist* nl, const double* p, const double* u,
+                   const double* du, double t, int mode, double* out);
+int    or_jacobian(const cs_netlist* nl, const double* p, const double* u,
+                   double gamma, double t, int mode, double* J);
+/* Charge vector Q(u) and its pattern flag per row (1 = row carries charge). */
+int    or_charges(const cs_netlist* nl, const double* p, const double* u,
+                  double* q, int* is_diff);
+
+/* Dense LU with partial pivoting; solve.  Returns 0 or CS_E_SINGULAR. */
+int    or_lu_factor(int n, double* A, int* piv);
+void   or_lu_solve(int n, const double* LU, const int* piv, double* b);
+
+int    or_dcop(const cs_netlist* nl, const double* p, const cs_tran_opts* o,
+               double* u_out, cs_tran_stats* st);
+int    or_tran(const cs_netlist* nl, const double* p, const double* u0,
+               const cs_tran_opts* o, int n_probe, const int* probe,
+               double* wave, double* u_final, cs_tran_stats* st);
+/* Many instances, OpenMP across instances (nthreads <= 0: all cores). */
+int    or_tran_batch(int n_inst, const cs_netlist* const* nls,
+                     const double* params, const long long* p_off,
+                     const double* u0, const long long* u_off,
+                     const cs_tran_opts* o, const int* n_probe,
+                     const int* const* probes, double* wave,
+                     const long long* w_off, double* u_final,
+                     cs_tran_stats* st, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""ctypes wrapper of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the parity checker for the HIP library and the CPU
+baseline timed by bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module.
+
+Inputs are flat netlists as plain attributes (n_nodes, arrays{...}, gmin) —
+the data format of include/cedarhip.h — so the oracle does not depend on the
+product's Python package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, Structure, byref, c_double, c_int, c_int32, c_int64, c_longlong
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_I = POINTER(c_int32)
+_D = POINTER(c_double)
+
+
+class _Netlist(Structure):
+    _fields_ = [
+        ("n_nodes", c_int32),
+        ("n_res", c_int32), ("res_nodes", _I), ("res_r", _D),
+        ("n_cap", c_int32), ("cap_nodes", _I), ("cap_c", _D),
+        ("n_ind", c_int32), ("ind_nodes", _I), ("ind_l", _D),
+        ("n_vsrc", c_int32), ("vsrc_nodes", _I), ("vsrc_wave", _I),
+        ("n_isrc", c_int32), ("isrc_nodes", _I), ("isrc_wave", _I),
+        ("n_vccs", c_int32), ("vccs_nodes", _I), ("vccs_gain", _D),
+        ("n_vcvs", c_int32), ("vcvs_nodes", _I), ("vcvs_gain", _D),
+        ("n_mos", c_int32), ("mos_nodes", _I), ("mos_type", _I),
+        ("mos_vth", _D), ("mos_kp", _D), ("mos_w", _D), ("mos_l", _D),
+        ("n_wave", c_int32), ("wave_kind", _I), ("wave_off", _I), ("wave_len", _I),
+        ("wave_dc", _D),
+        ("n_wave_data", c_int32), ("wave_data", _D),
+        ("gmin", c_double),
+    ]
+
+
+class _Opts(Structure):
+    _fields_ = [("t_start", c_double), ("t_stop", c_double), ("reltol", c_double),
+                ("abstol", c_double), ("dt_init", c_double), ("dt_min", c_double),
+                ("dt_max", c_double), ("newton_tol", c_double), ("dt_out", c_double),
+                ("dc_abstol", c_double), ("n_out", c_int32), ("max_order", c_int32),
+                ("newton_maxit", c_int32), ("init", c_int32), ("max_steps", c_int64),
+                ("lanes", c_int32), ("record_steps", c_int32)]
+
+
+class Stats(Structure):
+    _fields_ = [("steps", c_int64), ("rejected", c_int64), ("newton_iters", c_int64),
+                ("t_end", c_double), ("retcode", c_int32), ("pad", c_int32)]
+
+
+_OPT_DEFAULTS = dict(t_start=0.0, t_stop=1e-9, reltol=1e-6, abstol=1e-9, dt_init=0.0, dt_min=0.0,
+                     dt_max=0.0, newton_tol=1e-2, dt_out=0.0, dc_abstol=1e-10, n_out=0,
+                     max_order=2, newton_maxit=8, init=0, max_steps=10_000_000, lanes=0,
+                     record_steps=0)
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        N = POINTER(_Netlist)
+        L.or_wave_value.restype = c_double
+        L.or_wave_value.argtypes = [c_int, _D, c_int, c_double]
+        L.or_wave_next_bp.restype = c_double
+        L.or_wave_next_bp.argtypes = [c_int, _D, c_int, c_double]
+        L.or_pwl_at_time.restype = c_double
+        L.or_pwl_at_time.argtypes = [_D, c_int, c_double]
+        L.or_sqlaw.restype = None
+        L.or_sqlaw.argtypes = [c_int] + [c_double] * 8 + [_D] * 4
+        L.or_residual.argtypes = [N, _D, _D, _D, c_double, c_int, _D]
+        L.or_jacobian.argtypes = [N, _D, _D, c_double, c_double, c_int, _D]
+        L.or_lu_factor.argtypes = [c_int, _D, POINTER(c_int)]
+        L.or_lu_solve.argtypes = [c_int, _D, POINTER(c_int), _D]
+        L.or_lu_solve.restype = None
+        L.or_dcop.argtypes = [N, _D, POINTER(_Opts), _D, POINTER(Stats)]
+        L.or_tran.argtypes = [N, _D, _D, POINTER(_Opts), c_int, POINTER(c_int), _D, _D,
+                              POINTER(Stats)]
+        L.or_tran_batch.argtypes = [c_int, POINTER(N), _D, POINTER(c_longlong), _D,
+                                    POINTER(c_longlong), POINTER(_Opts), POINTER(c_int),
+                                    POINTER(POINTER(c_int)), _D, POINTER(c_longlong), _D,
+                                    POINTER(Stats), c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class Netlist:
+    """Keeps the arrays alive for the C struct."""
+
+    def __init__(self, flat):
+        a = {k: np.ascontiguousarray(v) for k, v in flat.arrays.items()}
+        self.keep = a
+        s = _Netlist()
+        s.n_nodes = flat.n_nodes
+        for dev, nodes, val in (("res", "res_nodes", "res_r"), ("cap", "cap_nodes", "cap_c"),
+                                ("ind", "ind_nodes", "ind_l")):
+            setattr(s, "n_" + dev, len(a[val]))
+            setattr(s, nodes, _ip(a[nodes]))
+            setattr(s, val, _dp(a[val]))
+        s.n_vsrc, s.vsrc_nodes, s.vsrc_wave = len(a["vsrc_wave"]), _ip(a["vsrc_nodes"]), _ip(a["vsrc_wave"])
+        s.n_isrc, s.isrc_nodes, s.isrc_wave = len(a["isrc_wave"]), _ip(a["isrc_nodes"]), _ip(a["isrc_wave"])
+        s.n_vccs, s.vccs_nodes, s.vccs_gain = len(a["vccs_gain"]), _ip(a["vccs_nodes"]), _dp(a["vccs_gain"])
+        s.n_vcvs, s.vcvs_nodes, s.vcvs_gain = len(a["vcvs_gain"]), _ip(a["vcvs_nodes"]), _dp(a["vcvs_gain"])
+        s.n_mos, s.mos_nodes, s.mos_type = len(a["mos_vth"]), _ip(a["mos_nodes"]), _ip(a["mos_type"])
+        s.mos_vth, s.mos_kp, s.mos_w, s.mos_l = (_dp(a["mos_vth"]), _dp(a["mos_kp"]),
+                                                  _dp(a["mos_w"]), _dp(a["mos_l"]))
+        s.n_wave = len(a["wave_kind"])
+        s.wave_kind, s.wave_off, s.wave_len = _ip(a["wave_kind"]), _ip(a["wave_off"]), _ip(a["wave_len"])
+        s.wave_dc = _dp(a["wave_dc"])
+        s.n_wave_data, s.wave_data = len(a["wave_data"]), _dp(a["wave_data"])
+        s.gmin = flat.gmin
+        self.s = s
+        self.n = (flat.n_nodes + len(a["vsrc_wave"]) + len(a["ind_l"]) + len(a["vcvs_gain"]))
+
+
+def opts(o=None, **kw) -> _Opts:
+    d = dict(_OPT_DEFAULTS)
+    if o is not None:
+        for k in _OPT_DEFAULTS:
+            if hasattr(o, k):
+                d[k] = getattr(o, k)
+            elif isinstance(o, dict) and k in o:
+                d[k] = o[k]
+    d.update(kw)
+    r = _Opts()
+    for k, v in d.items():
+        setattr(r, k, v)
+    return r
+
+
+def wave_value(kind, data, t):
+    d = np.ascontiguousarray(data, np.float64)
+    return lib().or_wave_value(kind, _dp(d), len(d), t)
+
+
+def wave_next_bp(kind, data, t):
+    d = np.ascontiguousarray(data, np.float64)
+    return lib().or_wave_next_bp(kind, _dp(d), len(d), t)
+
+
+def pwl_at_time(ts, ys, t):
+    d = np.ascontiguousarray(np.column_stack([ts, ys]).reshape(-1), np.float64)
+    return lib().or_pwl_at_time(_dp(d), len(ts), t)
+
+
+def sqlaw(type_, vth, kp, w, l, gmin, vd, vg, vs):
+    out = [c_double() for _ in range(4)]
+    lib().or_sqlaw(int(type_), vth, kp, w, l, gmin, vd, vg, vs, *[byref(x) for x in out])
+    return tuple(x.value for x in out)
+
+
+def residual(flat, p, u, du, t, mode=0):
+    nl = Netlist(flat)
+    out = np.zeros(nl.n)
+    du = None if du is None else np.ascontiguousarray(du, np.float64)
+    lib().or_residual(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                      _dp(np.ascontiguousarray(u, np.float64)), _dp(du), t, mode, _dp(out))
+    return out
+
+
+def jacobian(flat, p, u, gamma, t, mode=0):
+    nl = Netlist(flat)
+    J = np.zeros((nl.n, nl.n))
+    lib().or_jacobian(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                      _dp(np.ascontiguousarray(u, np.float64)), gamma, t, mode, _dp(J))
+    return J
+
+
+def lu_solve(A, b):
+    A = np.ascontiguousarray(A, np.float64).copy()
+    n = A.shape[0]
+    piv = (c_int * n)()
+    rc = lib().or_lu_factor(n, _dp(A), piv)
+    if rc != 0:
+        raise np.linalg.LinAlgError("singular")
+    x = np.ascontiguousarray(b, np.float64).copy()
+    lib().or_lu_solve(n, _dp(A), piv, _dp(x))
+    return x
+
+
+def dcop(flat, p, o=None, **kw):
+    nl = Netlist(flat)
+    u = np.zeros(nl.n)
+    st = Stats()
+    oo = opts(o, **kw)
+    rc = lib().or_dcop(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)), byref(oo),
+                       _dp(u), byref(st))
+    return u, st, rc
+
+
+def tran(flat, p, u0, o=None, probes=None, **kw):
+    nl = Netlist(flat)
+    oo = opts(o, **kw)
+    pr = np.arange(nl.n, dtype=np.int32) if probes is None else np.asarray(probes, np.int32)
+    pc = (c_int * len(pr))(*pr.tolist())
+    wave = np.zeros((max(oo.n_out, 0), len(pr)))
+    uf = np.zeros(nl.n)
+    st = Stats()
+    u0a = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+    rc = lib().or_tran(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)), _dp(u0a), byref(oo),
+                       len(pr), pc, _dp(wave), _dp(uf), byref(st))
+    return wave, uf, st, rc
+
+
+def tran_batch(flats, params, u0s, o=None, probes=None, nthreads=0, **kw):
+    """OpenMP across instances.  flats[i] may repeat the same object."""
+    nls_cache = {}
+    nls = []
+    for f in flats:
+        if id(f) not in nls_cache:
+            nls_cache[id(f)] = Netlist(f)
+        nls.append(nls_cache[id(f)])
+    n_inst = len(flats)
+    oo = opts(o, **kw)
+    ptrs = (POINTER(_Netlist) * n_inst)(*[ctypes.pointer(x.s) for x in nls])
+    p_all = np.ascontiguousarray(np.concatenate(params), np.float64)
+    p_off = np.cumsum([0] + [len(p) for p in params[:-1]]).astype(np.int64)
+    u_off = np.cumsum([0] + [x.n for x in nls[:-1]]).astype(np.int64)
+    u_all = None if u0s is None else np.ascontiguousarray(np.concatenate(u0s), np.float64)
+    prl = [np.arange(x.n, dtype=np.int32) if probes is None else np.asarray(probes[i], np.int32)
+           for i, x in enumerate(nls)]
+    pr_c = [(c_int * len(p))(*p.tolist()) for p in prl]
+    pr_ptrs = (POINTER(c_int) * n_inst)(*[ctypes.cast(p, POINTER(c_int)) for p in pr_c])
+    npb = (c_int * n_inst)(*[len(p) for p in prl])
+    w_sizes = [oo.n_out * len(p) for p in prl]
+    w_off = np.cumsum([0] + w_sizes[:-1]).astype(np.int64)
+    wave = np.zeros(max(sum(w_sizes), 1))
+    uf = np.zeros(int(sum(x.n for x in nls)))
+    st = (Stats * n_inst)()
+    rc = lib().or_tran_batch(n_inst, ptrs, _dp(p_all), p_off.ctypes.data_as(POINTER(c_longlong)),
+                             _dp(u_all), u_off.ctypes.data_as(POINTER(c_longlong)), byref(oo), npb,
+                             pr_ptrs, _dp(wave), w_off.ctypes.data_as(POINTER(c_longlong)), _dp(uf),
+                             st, nthreads)
+    waves = [wave[w_off[i]:w_off[i] + w_sizes[i]].reshape(oo.n_out, len(prl[i])) for i in range(n_inst)]
+    ufs = [uf[u_off[i]:u_off[i] + nls[i].n] for i in range(n_inst)]
+    return waves, ufs, list(st), rc
