@@ -1,0 +1,170 @@
+"""Benchmark: transient instances/s of the batched ring-oscillator Monte Carlo
+on one MI355X, next to the plain per-instance GPU path and the CPU oracle.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one "step" = one full batched transient over the whole batch):
+  4096 ring oscillators, stage count N uniform odd in [11, 101] (uneven sizes,
+  one launch), square-law inverters (NMOS Vth 0.4 V, KP 200 uA/V^2, W/L
+  1u/0.18u; PMOS Vth -0.4 V, KP 80 uA/V^2, W/L 2u/0.18u), per-transistor Vth
+  offset ~ N(0, 30 mV), 5 fF per stage, VDD 1.8 V, 0-10 ns from a seeded
+  perturbed initial state, BDF2 at reltol 1e-6 / abstol 1e-9 V, FP64.
+
+Inputs are uploaded before the timed region (they are resident in HBM when it
+starts); the timed region is cs_batch_run (launch + completion).  The plain
+per-instance path (host step control, one captured hipGraph launch per Newton
+iteration) is timed on a sample of instances over the full 0-10 ns span, and
+the CPU oracle (OpenMP across instances) on a bounded sample; both are
+reported as instances/s.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cedarsim.jl_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-inst", type=int, default=4096)
+    ap.add_argument("--t-stop", type=float, default=10e-9)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--plain-sample", type=int, default=1, help="instances timed on the plain path")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="instances timed on the CPU oracle")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import cedarhip as ch
+    from cedarhip import lib as L
+    from cedarhip.workloads import ring_mc
+
+    log(f"building workload: {args.n_inst} rings, seed {args.seed}")
+    flats, params, u0s, stages = ring_mc(n_inst=args.n_inst, seed=args.seed)
+    opts = ch.TranOptions(t_start=0.0, t_stop=args.t_stop, reltol=1e-6, abstol=1e-9,
+                          init=L.INIT_UIC, max_order=2, n_out=0, dt_out=0.0)
+    eng = ch.Engine(0)
+    systems, key, inst_sys = [], {}, []
+    for f in flats:
+        if id(f) not in key:
+            key[id(f)] = len(systems)
+            systems.append(eng.load(f))
+        inst_sys.append(key[id(f)])
+    log(f"{len(systems)} distinct circuits compiled; uploading batch")
+    batch = ch.Batch(eng, systems, inst_sys, params, opts, u0s)
+    for w in range(args.warmup):
+        ms = batch.run()
+        log(f"warmup {w}: {ms:.1f} ms")
+    times = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ms = batch.run()
+        times.append(ms)
+        log(f"step {k}: {ms:.1f} ms")
+    wall = time.perf_counter() - t0
+    res = batch.results(times[-1])
+    bad = [i for i, s in enumerate(res.stats) if s.retcode != 0]
+    if bad:
+        log(f"{len(bad)} instances failed: first {bad[:5]} {[res.retcodes[i] for i in bad[:5]]}")
+    ms_per_step = wall / args.steps * 1e3
+    value = args.n_inst * args.steps / wall
+    kernel_ms = float(np.mean(times))
+    iters = sum(s.newton_iters for s in res.stats)
+    steps_acc = sum(s.steps for s in res.stats)
+    n_mos_tot = sum(len(systems[inst_sys[i]].flat.arrays["mos_vth"]) * res.stats[i].newton_iters
+                    for i in range(args.n_inst))
+    achieved = res.alg_bytes / (kernel_ms * 1e-3)
+    out = {
+        "metric": "transient instances/sec at reltol 1e-6, batched vs plain (1 GPU); device evals/s",
+        "value": value,
+        "unit": "instances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"ring-oscillator Monte Carlo: {args.n_inst} rings, N odd U[11,101], "
+                         f"sigma_vth 30 mV, 5 fF/stage, VDD 1.8 V, 0-{args.t_stop * 1e9:g} ns, "
+                         "BDF2 reltol 1e-6 abstol 1e-9 V"),
+            "n_inst": args.n_inst, "seed": args.seed,
+            "newton_iterations": int(iters), "accepted_steps": int(steps_acc),
+            "failed_instances": len(bad),
+        },
+        "device_evals_per_s": n_mos_tot / (kernel_ms * 1e-3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "kernel": "k_tran_batch",
+            "kernel_ms": kernel_ms,
+            "alg_bytes_per_launch": res.alg_bytes,
+            "alg_flops_per_launch": res.alg_flops,
+        },
+    }
+    batch.close()
+
+    if not args.no_plain and args.plain_sample > 0:
+        # plain per-instance path over the full span, on the first instances
+        idx = list(range(args.plain_sample))
+        t0 = time.perf_counter()
+        for i in idx:
+            s = systems[inst_sys[i]]
+            sol = s.tran(opts, params[i], u0s[i])
+            if sol.retcode != "Success":
+                log(f"plain instance {i} failed: {sol.retcode}")
+        dt = time.perf_counter() - t0
+        plain = len(idx) / dt
+        out["plain"] = {"value": plain, "unit": "instances/s",
+                        "sample": f"{len(idx)} instance(s) (N={[int(stages[i]) for i in idx]}), "
+                                  f"full 0-{args.t_stop * 1e9:g} ns each"}
+        out["batched_vs_plain"] = value / plain
+        log(f"plain: {plain:.4f} inst/s ({dt:.1f} s)")
+
+    if not args.no_cpu and args.cpu_sample > 0:
+        from oracle import oracle as O
+        k = min(args.cpu_sample, args.n_inst)
+        thr = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        O.set_sparse(True)
+        t0 = time.perf_counter()
+        w, uf, st, rc = O.tran_batch(flats[:k], params[:k], u0s[:k], opts, nthreads=thr)
+        dt = time.perf_counter() - t0
+        O.set_sparse(False)
+        out["cpu_baseline"] = {"value": k / dt, "unit": "instances/s", "cores": thr, "kind": "port",
+                               "sample": f"first {k} instances of the same seeded batch, full "
+                                         f"0-{args.t_stop * 1e9:g} ns, OpenMP across instances, "
+                                         f"sparse left-looking LU (oracle/sparse.c)"}
+        log(f"cpu: {k / dt:.3f} inst/s ({dt:.1f} s, {thr} threads, rc {rc})")
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

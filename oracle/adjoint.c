@@ -1,0 +1,385 @@
+/*
+ * adjoint.c — discrete adjoint of the oracle's BDF transient (test
+ * infrastructure and CPU baseline only; see oracle.h).
+ *
+ * Objective: energy delivered by voltage source s over the run,
+ *     E = sum_n w_n P(u_n),  P(u) = -(V(a) - V(b)) * I_s,
+ * with trapezoid weights w_n on the accepted time points (so E is exactly
+ * what the forward run integrates).  Parameters: all MOSFET widths W_k.
+ *
+ * Forward: the adaptive run of oracle.c (UIC start), every accepted state and
+ * BDF coefficient kept in memory.  Step n solves
+ *     G_n(u_n) = F(u_n, t_n) + a_n0 Q(u_n) + a_n1 Q(u_{n-1}) + a_n2 Q(u_{n-2}) = 0
+ * and step 0 is the consistent-initialisation system (charge rows pinned to
+ * u0, algebraic rows F_r = 0).
+ * Reverse (step sequence frozen, solves taken as exact):
+ *     J_n^T l_n = -( w_n dP/du(u_n) + a_{n+1,1} C^T l_{n+1} + a_{n+2,2} C^T l_{n+2} )
+ *     dE/dW_k   = sum_n l_n^T dG_n/dW_k,   dG_n/dW_k = +/- I_ch,k(u_n) / W_k
+ *                 on the drain / source rows (I_ch excludes the gmin part).
+ * Replay: the same step sequence and predictor segments, Newton to a tight
+ * tolerance, for central finite differences of E in W.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+typedef struct {
+    double t, a0, a1, a2;
+    int k, npts;
+} step_rec;
+
+static int nunk(const cs_netlist* nl) { return or_num_unknowns(nl); }
+
+/* dP/du for source s and P itself */
+static double power(const cs_netlist* nl, int s, const double* u, double* dP) {
+    int a = nl->vsrc_nodes[2 * s], b = nl->vsrc_nodes[2 * s + 1];
+    int j = nl->n_nodes + s;
+    double va = a >= 0 ? u[a] : 0.0, vb = b >= 0 ? u[b] : 0.0;
+    if (dP) {
+        int n = nunk(nl);
+        memset(dP, 0, sizeof(double) * n);
+        if (a >= 0) dP[a] += -u[j];
+        if (b >= 0) dP[b] += u[j];
+        dP[j] += -(va - vb);
+    }
+    return -(va - vb) * u[j];
+}
+
+/* Newton solve of one BDF step (dense): u in/out */
+static int step_solve(const cs_netlist* nl, const double* p, int n, double tn, double a0,
+                      const double* qc, double tol_abs_rel[2], double ntol, int maxit,
+                      double* u, double* F, double* Q, double* J, double* C, double* d,
+                      int* piv, int64_t* nit) {
+    for (int it = 0; it < maxit; it++) {
+        or_residual(nl, p, u, NULL, tn, CS_MODE_TRAN, F);
+        or_charges(nl, p, u, Q, NULL);
+        or_jacobian(nl, p, u, a0, tn, CS_MODE_TRAN, J);
+        for (int i = 0; i < n; i++) d[i] = -(F[i] + a0 * Q[i] + qc[i]);
+        if (nit) (*nit)++;
+        if (or_lu_factor(n, J, piv) != CS_OK) return CS_E_SINGULAR;
+        or_lu_solve(n, J, piv, d);
+        double s = 0.0;
+        for (int i = 0; i < n; i++) {
+            u[i] += d[i];
+            double r = d[i] / (tol_abs_rel[1] * fabs(u[i]) + tol_abs_rel[0]);
+            s += r * r;
+        }
+        if (!(s == s)) return CS_E_NOCONV;
+        if (sqrt(s / n) <= ntol) return CS_OK;
+    }
+    return CS_E_NOCONV;
+}
+
+/* consistent initial point under UIC (dense Newton) */
+static int init_uic(const cs_netlist* nl, const double* p, int n, double t0, double tol,
+                    const double* u0, double* u, int* is_diff, double* F, double* J,
+                    double* d, int* piv) {
+    double* q = (double*)malloc(sizeof(double) * n);
+    memcpy(u, u0, sizeof(double) * n);
+    or_charges(nl, p, u, q, is_diff);
+    free(q);
+    for (int it = 0; it < 50; it++) {
+        or_residual(nl, p, u, NULL, t0, CS_MODE_TRAN, F);
+        or_jacobian(nl, p, u, 0.0, t0, CS_MODE_TRAN, J);
+        double fn = 0.0;
+        for (int r = 0; r < n; r++) {
+            if (is_diff[r]) {
+                F[r] = u[r] - u0[r];
+                for (int c = 0; c < n; c++) J[(size_t)r * n + c] = 0.0;
+                J[(size_t)r * n + r] = 1.0;
+            }
+            if (fabs(F[r]) > fn) fn = fabs(F[r]);
+        }
+        if (fn <= tol) return CS_OK;
+        for (int i = 0; i < n; i++) d[i] = -F[i];
+        if (or_lu_factor(n, J, piv) != CS_OK) return CS_E_SINGULAR;
+        or_lu_solve(n, J, piv, d);
+        for (int i = 0; i < n; i++) u[i] += d[i];
+    }
+    return CS_E_NOCONV;
+}
+
+/* d I_ch / d W for every MOSFET at state u */
+static void mos_dw(const cs_netlist* nl, const double* p, const double* u, double* g) {
+    cs_param_layout L;
+    or_layout(nl, &L);
+    for (int k = 0; k < nl->n_mos; k++) {
+        const int* nd = nl->mos_nodes + 4 * k;
+        double vd = nd[0] >= 0 ? u[nd[0]] : 0.0, vg = nd[1] >= 0 ? u[nd[1]] : 0.0;
+        double vs = nd[2] >= 0 ? u[nd[2]] : 0.0;
+        double id, a, b, c;
+        or_sqlaw(nl->mos_type[k], p[L.mos_vth + k], p[L.mos_kp + k], p[L.mos_w + k],
+                 p[L.mos_l + k], 0.0, vd, vg, vs, &id, &a, &b, &c);
+        g[k] = id / p[L.mos_w + k];
+    }
+}
+
+/* Adaptive forward with every state kept, then the reverse sweep. */
+int or_adjoint(const cs_netlist* nl, const double* p, const double* u0, const cs_tran_opts* o,
+               int vsrc, double* E_out, double* grad, cs_tran_stats* st_out,
+               long long* n_steps, double* t_log, int* npts_log, long long log_cap) {
+    const int n = nunk(nl);
+    const double span = o->t_stop - o->t_start;
+    const double dt_init = o->dt_init > 0 ? o->dt_init : span * 1e-9;
+    const double dt_min = o->dt_min > 0 ? o->dt_min : span * 1e-15;
+    const double dt_max = o->dt_max > 0 ? o->dt_max : span / 10.0;
+    double tols[2] = {o->abstol, o->reltol};
+    cs_tran_stats st;
+    memset(&st, 0, sizeof(st));
+    int cap = 1 << 16, M = 0;
+    double* US = (double*)malloc(sizeof(double) * (size_t)cap * n);
+    step_rec* SR = (step_rec*)malloc(sizeof(step_rec) * cap);
+    double *F = malloc(sizeof(double) * n), *Q = malloc(sizeof(double) * n);
+    double *J = malloc(sizeof(double) * n * n), *C = malloc(sizeof(double) * n * n);
+    double *d = malloc(sizeof(double) * n), *u = malloc(sizeof(double) * n);
+    double *up = malloc(sizeof(double) * n), *qc = malloc(sizeof(double) * n);
+    double *QH = malloc(sizeof(double) * 3 * n);
+    int* piv = malloc(sizeof(int) * n);
+    int* is_diff = malloc(sizeof(int) * n);
+    int rc = init_uic(nl, p, n, o->t_start, o->dc_abstol, u0, u, is_diff, F, J, d, piv);
+    if (rc != CS_OK) goto out;
+    memcpy(US, u, sizeof(double) * n);
+    SR[0].t = o->t_start; SR[0].k = 0; SR[0].npts = 0;
+    SR[0].a0 = SR[0].a1 = SR[0].a2 = 0.0;
+    or_charges(nl, p, u, QH, NULL);
+    int npts = 1;
+    double t = o->t_start, h = dt_init;
+    /* the state history for the predictor lives in US[M-2..M] */
+    while (t < o->t_stop) {
+        if (st.steps + st.rejected >= o->max_steps) { rc = CS_E_MAXSTEPS; break; }
+        double tb = or_next_breakpoint(nl, p, t), hh = h < dt_max ? h : dt_max;
+        int src_bp = 1;
+        if (!(tb < o->t_stop)) { tb = o->t_stop; src_bp = 0; }
+        int hit = 0;
+        if (tb - t <= hh * (1.0 + 1e-9)) { hh = tb - t; hit = 1; }
+        else if (tb - t < 2.0 * hh) hh = 0.5 * (tb - t);
+        double tn = hit ? tb : t + hh;
+        hh = tn - t;
+        int k = (npts >= 3 && o->max_order >= 2) ? 2 : 1;
+        double T0 = SR[M].t, T1 = M >= 1 ? SR[M - 1].t : 0.0, T2 = M >= 2 ? SR[M - 2].t : 0.0;
+        double a0, a1, a2 = 0.0;
+        if (k == 1) { a0 = 1.0 / (tn - T0); a1 = -a0; }
+        else {
+            a0 = 1.0 / (tn - T0) + 1.0 / (tn - T1);
+            a1 = (tn - T1) / ((T0 - tn) * (T0 - T1));
+            a2 = (tn - T0) / ((T1 - tn) * (T1 - T0));
+        }
+        int np = npts < k + 1 ? npts : k + 1;
+        const double* U0 = US + (size_t)M * n;
+        const double* U1 = M >= 1 ? US + (size_t)(M - 1) * n : U0;
+        const double* U2 = M >= 2 ? US + (size_t)(M - 2) * n : U0;
+        for (int i = 0; i < n; i++) {
+            if (np == 1) up[i] = U0[i];
+            else if (np == 2) up[i] = U0[i] + (tn - T0) / (T0 - T1) * (U0[i] - U1[i]);
+            else {
+                double l0 = (tn - T1) * (tn - T2) / ((T0 - T1) * (T0 - T2));
+                double l1 = (tn - T0) * (tn - T2) / ((T1 - T0) * (T1 - T2));
+                double l2 = (tn - T0) * (tn - T1) / ((T2 - T0) * (T2 - T1));
+                up[i] = l0 * U0[i] + l1 * U1[i] + l2 * U2[i];
+            }
+            qc[i] = a1 * QH[i] + (k == 2 ? a2 * QH[n + i] : 0.0);
+        }
+        memcpy(u, up, sizeof(double) * n);
+        int r2 = step_solve(nl, p, n, tn, a0, qc, tols, o->newton_tol, o->newton_maxit, u, F, Q,
+                            J, C, d, piv, &st.newton_iters);
+        if (r2 != CS_OK) {
+            st.rejected++;
+            h = 0.25 * hh;
+            if (h < dt_min) { rc = CS_E_DTMIN; break; }
+            continue;
+        }
+        double fac = 2.0;
+        if (npts >= 2) {
+            double c;
+            if (k == 1) c = (a1 * (T0 - tn) * (T0 - tn)) / (a0 * (tn - T0) * (tn - T1));
+            else {
+                double d1 = T0 - tn, d2 = T1 - tn;
+                c = (a1 * d1 * d1 * d1 + a2 * d2 * d2 * d2) / (a0 * (tn - T0) * (tn - T1) * (tn - T2));
+            }
+            double s = 0.0;
+            for (int i = 0; i < n; i++) {
+                double m = fabs(u[i]) > fabs(U0[i]) ? fabs(u[i]) : fabs(U0[i]);
+                double r = c * (u[i] - up[i]) / (o->reltol * m + o->abstol);
+                s += r * r;
+            }
+            double err = sqrt(s / n);
+            double ex = -1.0 / (double)(k + 1);
+            if (!(err <= 1.0)) {
+                st.rejected++;
+                double f = (err == err) ? 0.9 * pow(err, ex) : 0.1;
+                h = hh * (f < 0.1 ? 0.1 : f);
+                if (h < dt_min) { rc = CS_E_DTMIN; break; }
+                continue;
+            }
+            fac = (err > 0.0) ? 0.9 * pow(err, ex) : 2.0;
+            if (fac > 2.0) fac = 2.0;
+        }
+        if (M + 1 >= cap) {
+            cap *= 2;
+            US = (double*)realloc(US, sizeof(double) * (size_t)cap * n);
+            SR = (step_rec*)realloc(SR, sizeof(step_rec) * cap);
+        }
+        M++;
+        memcpy(US + (size_t)M * n, u, sizeof(double) * n);
+        SR[M].t = tn; SR[M].k = k; SR[M].npts = npts;
+        SR[M].a0 = a0; SR[M].a1 = a1; SR[M].a2 = a2;
+        memmove(QH + n, QH, sizeof(double) * n);
+        or_charges(nl, p, u, QH, NULL);
+        if (npts < 3) npts++;
+        st.steps++;
+        t = tn;
+        h = hh * fac;
+        if (hit && src_bp) {
+            npts = 1;
+            h = 0.1 * h;
+            if (h < dt_min) h = dt_min;
+        }
+    }
+    st.t_end = t;
+    if (rc != CS_OK) goto out;
+    /* objective */
+    {
+        double E = 0.0;
+        for (int m = 1; m <= M; m++)
+            E += 0.5 * (SR[m].t - SR[m - 1].t) * (power(nl, vsrc, US + (size_t)m * n, NULL) +
+                                                   power(nl, vsrc, US + (size_t)(m - 1) * n, NULL));
+        *E_out = E;
+    }
+    if (n_steps) *n_steps = M;
+    if (t_log && npts_log)
+        for (int m = 0; m <= M && m < log_cap; m++) { t_log[m] = SR[m].t; npts_log[m] = SR[m].npts; }
+    /* reverse sweep */
+    if (grad) {
+        double* lam = (double*)calloc((size_t)(M + 3) * n, sizeof(double));
+        double* rhs = malloc(sizeof(double) * n);
+        double* dP = malloc(sizeof(double) * n);
+        double* g = malloc(sizeof(double) * (nl->n_mos > 0 ? nl->n_mos : 1));
+        double* Cm = malloc(sizeof(double) * n * n);
+        cs_param_layout L;
+        or_layout(nl, &L);
+        memset(grad, 0, sizeof(double) * nl->n_mos);
+        for (int m = M; m >= 0; m--) {
+            const double* um = US + (size_t)m * n;
+            double wm = 0.0;
+            if (m > 0) wm += 0.5 * (SR[m].t - SR[m - 1].t);
+            if (m < M) wm += 0.5 * (SR[m + 1].t - SR[m].t);
+            power(nl, vsrc, um, dP);
+            for (int i = 0; i < n; i++) rhs[i] = -wm * dP[i];
+            /* C = dQ/du at u_m, computed directly (forming (G + C) - G would
+             * cancel catastrophically where G >> C) */
+            or_cmatrix(nl, p, um, Cm);
+            for (int q = 1; q <= 2; q++) {
+                int mm = m + q;
+                if (mm > M) break;
+                double coef = q == 1 ? SR[mm].a1 : SR[mm].a2;
+                if (coef == 0.0) continue;
+                const double* lq = lam + (size_t)mm * n;
+                for (int c = 0; c < n; c++) {
+                    double s = 0.0;
+                    for (int r = 0; r < n; r++) s += Cm[(size_t)r * n + c] * lq[r];
+                    rhs[c] -= coef * s;
+                }
+            }
+            /* J_m^T */
+            if (m > 0) {
+                or_jacobian(nl, p, um, SR[m].a0, SR[m].t, CS_MODE_TRAN, J);
+            } else {
+                or_jacobian(nl, p, um, 0.0, SR[0].t, CS_MODE_TRAN, J);
+                for (int r = 0; r < n; r++)
+                    if (is_diff[r]) {
+                        for (int c = 0; c < n; c++) J[(size_t)r * n + c] = 0.0;
+                        J[(size_t)r * n + r] = 1.0;
+                    }
+            }
+            for (int r = 0; r < n; r++)          /* transpose in place */
+                for (int c = r + 1; c < n; c++) {
+                    double tmp = J[(size_t)r * n + c];
+                    J[(size_t)r * n + c] = J[(size_t)c * n + r];
+                    J[(size_t)c * n + r] = tmp;
+                }
+            if (or_lu_factor(n, J, piv) != CS_OK) { rc = CS_E_SINGULAR; break; }
+            or_lu_solve(n, J, piv, rhs);
+            double* lm = lam + (size_t)m * n;
+            memcpy(lm, rhs, sizeof(double) * n);
+            mos_dw(nl, p, um, g);
+            for (int kq = 0; kq < nl->n_mos; kq++) {
+                int dn = nl->mos_nodes[4 * kq], sn = nl->mos_nodes[4 * kq + 2];
+                double ld = (dn >= 0 && !(m == 0 && is_diff[dn])) ? lm[dn] : 0.0;
+                double ls = (sn >= 0 && !(m == 0 && is_diff[sn])) ? lm[sn] : 0.0;
+                grad[kq] += (ld - ls) * g[kq];
+            }
+        }
+        free(lam); free(rhs); free(dP); free(g); free(Cm);
+    }
+out:
+    st.retcode = rc;
+    if (st_out) *st_out = st;
+    free(US); free(SR); free(F); free(Q); free(J); free(C); free(d); free(u); free(up);
+    free(qc); free(QH); free(piv); free(is_diff);
+    return rc;
+}
+
+/* Energy of a run that replays a recorded step sequence (times t_log[0..M],
+ * segment counts npts_log[1..M]) with Newton converged to ntol. */
+int or_replay_energy(const cs_netlist* nl, const double* p, const double* u0,
+                     const cs_tran_opts* o, int vsrc, long long M, const double* t_log,
+                     const int* npts_log, double ntol, double* E_out) {
+    const int n = nunk(nl);
+    double tols[2] = {o->abstol, o->reltol};
+    double *F = malloc(sizeof(double) * n), *Q = malloc(sizeof(double) * n);
+    double *J = malloc(sizeof(double) * n * n), *C = NULL;
+    double *d = malloc(sizeof(double) * n), *up = malloc(sizeof(double) * n);
+    double *qc = malloc(sizeof(double) * n), *QH = malloc(sizeof(double) * 2 * n);
+    double* U = malloc(sizeof(double) * 3 * n);      /* U0, U1, U2 */
+    int* piv = malloc(sizeof(int) * n);
+    int* is_diff = malloc(sizeof(int) * n);
+    double* u = malloc(sizeof(double) * n);
+    int rc = init_uic(nl, p, n, o->t_start, o->dc_abstol, u0, u, is_diff, F, J, d, piv);
+    double E = 0.0;
+    if (rc == CS_OK) {
+        memcpy(U, u, sizeof(double) * n);
+        or_charges(nl, p, u, QH, NULL);
+        double Pprev = power(nl, vsrc, u, NULL);
+        for (long long m = 1; m <= M && rc == CS_OK; m++) {
+            int npts = npts_log[m];
+            int k = (npts >= 3 && o->max_order >= 2) ? 2 : 1;
+            double tn = t_log[m], T0 = t_log[m - 1];
+            double T1 = m >= 2 ? t_log[m - 2] : 0.0, T2 = m >= 3 ? t_log[m - 3] : 0.0;
+            double a0, a1, a2 = 0.0;
+            if (k == 1) { a0 = 1.0 / (tn - T0); a1 = -a0; }
+            else {
+                a0 = 1.0 / (tn - T0) + 1.0 / (tn - T1);
+                a1 = (tn - T1) / ((T0 - tn) * (T0 - T1));
+                a2 = (tn - T0) / ((T1 - tn) * (T1 - T0));
+            }
+            int np = npts < k + 1 ? npts : k + 1;
+            for (int i = 0; i < n; i++) {
+                double u0i = U[i], u1i = U[n + i], u2i = U[2 * n + i];
+                if (np == 1) up[i] = u0i;
+                else if (np == 2) up[i] = u0i + (tn - T0) / (T0 - T1) * (u0i - u1i);
+                else {
+                    double l0 = (tn - T1) * (tn - T2) / ((T0 - T1) * (T0 - T2));
+                    double l1 = (tn - T0) * (tn - T2) / ((T1 - T0) * (T1 - T2));
+                    double l2 = (tn - T0) * (tn - T1) / ((T2 - T0) * (T2 - T1));
+                    up[i] = l0 * u0i + l1 * u1i + l2 * u2i;
+                }
+                qc[i] = a1 * QH[i] + (k == 2 ? a2 * QH[n + i] : 0.0);
+            }
+            memcpy(u, up, sizeof(double) * n);
+            rc = step_solve(nl, p, n, tn, a0, qc, tols, ntol, 30, u, F, Q, J, C, d, piv, NULL);
+            memmove(U + n, U, sizeof(double) * 2 * n);
+            memcpy(U, u, sizeof(double) * n);
+            memmove(QH + n, QH, sizeof(double) * n);
+            or_charges(nl, p, u, QH, NULL);
+            double Pn = power(nl, vsrc, u, NULL);
+            E += 0.5 * (tn - T0) * (Pn + Pprev);
+            Pprev = Pn;
+        }
+    }
+    *E_out = E;
+    free(F); free(Q); free(J); free(d); free(up); free(qc); free(QH); free(U); free(piv);
+    free(is_diff); free(u);
+    return rc;
+}

@@ -44,6 +44,7 @@
  *             segment (quadratic with 3 points, else linear).
  */
 #include "oracle.h"
+#include "sparse.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -243,16 +244,43 @@ void or_sqlaw(int type, double vth, double kp, double w, double l, double gmin,
 /* MNA load: F (currents + branch equations), Q (charges), G = dF/du,    */
 /* C = dQ/du (dense, row-major).  Any output may be NULL.               */
 /* ------------------------------------------------------------------ */
+/* Matrix outputs go to dense n x n arrays (G, C), to sparse value arrays
+ * (Gs, Cs) addressed through the recorded call order (gmap, cmap), or are
+ * recorded as (i, j) pairs to build the sparse pattern. */
 typedef struct {
     int n;
     double *F, *Q, *G, *C;
+    double *Gs, *Cs;
+    const int *gmap, *cmap;
+    int gc, cc;
+    int *rec_g, *rec_c;      /* growable pair lists when recording */
+    int ng, nc, capg, capc;
+    int record;
 } load_out;
 
 static inline void addv(double* v, int i, double x) {
     if (v && i >= 0) v[i] += x;
 }
-static inline void addm(double* M, int n, int i, int j, double x) {
-    if (M && i >= 0 && j >= 0) M[(size_t)i * n + j] += x;
+static void rec_push(int** a, int* len, int* cap, int i, int j) {
+    if (*len == *cap) {
+        *cap = *cap ? 2 * *cap : 64;
+        *a = (int*)realloc(*a, sizeof(int) * 2 * *cap);
+    }
+    (*a)[2 * *len] = i;
+    (*a)[2 * *len + 1] = j;
+    (*len)++;
+}
+static inline void addG(load_out* o, int i, int j, double x) {
+    if (i < 0 || j < 0) return;
+    if (o->G) o->G[(size_t)i * o->n + j] += x;
+    if (o->Gs) o->Gs[o->gmap[o->gc++]] += x;
+    if (o->record) rec_push(&o->rec_g, &o->ng, &o->capg, i, j);
+}
+static inline void addC(load_out* o, int i, int j, double x) {
+    if (i < 0 || j < 0) return;
+    if (o->C) o->C[(size_t)i * o->n + j] += x;
+    if (o->Cs) o->Cs[o->cmap[o->cc++]] += x;
+    if (o->record) rec_push(&o->rec_c, &o->nc, &o->capc, i, j);
 }
 
 static void mna_load(const cs_netlist* nl, const double* p, const double* u,
@@ -264,6 +292,7 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
     if (o->Q) memset(o->Q, 0, sizeof(double) * n);
     if (o->G) memset(o->G, 0, sizeof(double) * n * n);
     if (o->C) memset(o->C, 0, sizeof(double) * n * n);
+    o->gc = o->cc = 0;    /* sparse values are zeroed by the caller */
 #define VV(i) ((i) < 0 ? 0.0 : u[i])
     const int bv = nl->n_nodes, bl = bv + nl->n_vsrc, be = bl + nl->n_ind;
     for (int k = 0; k < nl->n_res; k++) {
@@ -271,16 +300,16 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
         double g = 1.0 / p[L.res_r + k];
         double i = g * (VV(a) - VV(b));
         addv(o->F, a, i); addv(o->F, b, -i);
-        addm(o->G, n, a, a, g); addm(o->G, n, a, b, -g);
-        addm(o->G, n, b, a, -g); addm(o->G, n, b, b, g);
+        addG(o, a, a, g); addG(o, a, b, -g);
+        addG(o, b, a, -g); addG(o, b, b, g);
     }
     for (int k = 0; k < nl->n_cap; k++) {
         int a = nl->cap_nodes[2 * k], b = nl->cap_nodes[2 * k + 1];
         double c = p[L.cap_c + k];
         double q = c * (VV(a) - VV(b));
         addv(o->Q, a, q); addv(o->Q, b, -q);
-        addm(o->C, n, a, a, c); addm(o->C, n, a, b, -c);
-        addm(o->C, n, b, a, -c); addm(o->C, n, b, b, c);
+        addC(o, a, a, c); addC(o, a, b, -c);
+        addC(o, b, a, -c); addC(o, b, b, c);
     }
     for (int k = 0; k < nl->n_ind; k++) {
         int a = nl->ind_nodes[2 * k], b = nl->ind_nodes[2 * k + 1], j = bl + k;
@@ -288,17 +317,17 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
         addv(o->F, a, I); addv(o->F, b, -I);
         addv(o->F, j, VV(a) - VV(b));
         addv(o->Q, j, -l * I);
-        addm(o->G, n, a, j, 1.0); addm(o->G, n, b, j, -1.0);
-        addm(o->G, n, j, a, 1.0); addm(o->G, n, j, b, -1.0);
-        addm(o->C, n, j, j, -l);
+        addG(o, a, j, 1.0); addG(o, b, j, -1.0);
+        addG(o, j, a, 1.0); addG(o, j, b, -1.0);
+        addC(o, j, j, -l);
     }
     for (int k = 0; k < nl->n_vsrc; k++) {
         int a = nl->vsrc_nodes[2 * k], b = nl->vsrc_nodes[2 * k + 1], j = bv + k;
         double E = src_value(nl, p, &L, nl->vsrc_wave[k], t, mode), I = u[j];
         addv(o->F, a, I); addv(o->F, b, -I);
         addv(o->F, j, VV(a) - VV(b) - E);
-        addm(o->G, n, a, j, 1.0); addm(o->G, n, b, j, -1.0);
-        addm(o->G, n, j, a, 1.0); addm(o->G, n, j, b, -1.0);
+        addG(o, a, j, 1.0); addG(o, b, j, -1.0);
+        addG(o, j, a, 1.0); addG(o, j, b, -1.0);
     }
     for (int k = 0; k < nl->n_isrc; k++) {
         int a = nl->isrc_nodes[2 * k], b = nl->isrc_nodes[2 * k + 1];
@@ -310,8 +339,8 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
         double gm = p[L.vccs_gain + k];
         double i = gm * (VV(nd[2]) - VV(nd[3]));
         addv(o->F, nd[0], i); addv(o->F, nd[1], -i);
-        addm(o->G, n, nd[0], nd[2], gm); addm(o->G, n, nd[0], nd[3], -gm);
-        addm(o->G, n, nd[1], nd[2], -gm); addm(o->G, n, nd[1], nd[3], gm);
+        addG(o, nd[0], nd[2], gm); addG(o, nd[0], nd[3], -gm);
+        addG(o, nd[1], nd[2], -gm); addG(o, nd[1], nd[3], gm);
     }
     for (int k = 0; k < nl->n_vcvs; k++) {
         const int* nd = nl->vcvs_nodes + 4 * k;
@@ -319,9 +348,9 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
         double gn = p[L.vcvs_gain + k], I = u[j];
         addv(o->F, nd[0], I); addv(o->F, nd[1], -I);
         addv(o->F, j, VV(nd[0]) - VV(nd[1]) - gn * (VV(nd[2]) - VV(nd[3])));
-        addm(o->G, n, nd[0], j, 1.0); addm(o->G, n, nd[1], j, -1.0);
-        addm(o->G, n, j, nd[0], 1.0); addm(o->G, n, j, nd[1], -1.0);
-        addm(o->G, n, j, nd[2], -gn); addm(o->G, n, j, nd[3], gn);
+        addG(o, nd[0], j, 1.0); addG(o, nd[1], j, -1.0);
+        addG(o, j, nd[0], 1.0); addG(o, j, nd[1], -1.0);
+        addG(o, j, nd[2], -gn); addG(o, j, nd[3], gn);
     }
     double gmin = p[L.gmin];
     for (int k = 0; k < nl->n_mos; k++) {
@@ -332,8 +361,8 @@ static void mna_load(const cs_netlist* nl, const double* p, const double* u,
                  p[L.mos_w + k], p[L.mos_l + k], gmin, VV(d), VV(g), VV(s),
                  &id, &dd, &dg, &ds);
         addv(o->F, d, id); addv(o->F, s, -id);
-        addm(o->G, n, d, d, dd); addm(o->G, n, d, g, dg); addm(o->G, n, d, s, ds);
-        addm(o->G, n, s, d, -dd); addm(o->G, n, s, g, -dg); addm(o->G, n, s, s, -ds);
+        addG(o, d, d, dd); addG(o, d, g, dg); addG(o, d, s, ds);
+        addG(o, s, d, -dd); addG(o, s, g, -dg); addG(o, s, s, -ds);
     }
 #undef VV
 }
@@ -343,7 +372,8 @@ int or_residual(const cs_netlist* nl, const double* p, const double* u,
     int n = or_num_unknowns(nl);
     double* C = (double*)malloc(sizeof(double) * n * n);
     if (!C) return CS_E_OOM;
-    load_out o = {n, out, NULL, NULL, C};
+    load_out o = {0};
+    o.n = n; o.F = out; o.C = C;
     mna_load(nl, p, u, t, mode, &o);
     if (du && mode == CS_MODE_TRAN)
         for (int i = 0; i < n; i++)
@@ -357,7 +387,8 @@ int or_jacobian(const cs_netlist* nl, const double* p, const double* u,
     int n = or_num_unknowns(nl);
     double* C = (double*)malloc(sizeof(double) * n * n);
     if (!C) return CS_E_OOM;
-    load_out o = {n, NULL, NULL, J, C};
+    load_out o = {0};
+    o.n = n; o.G = J; o.C = C;
     mna_load(nl, p, u, t, mode, &o);
     if (mode == CS_MODE_TRAN)
         for (size_t i = 0; i < (size_t)n * n; i++) J[i] += gamma * C[i];
@@ -365,10 +396,20 @@ int or_jacobian(const cs_netlist* nl, const double* p, const double* u,
     return CS_OK;
 }
 
+/* dQ/du alone (dense row-major), without forming G + C and subtracting */
+int or_cmatrix(const cs_netlist* nl, const double* p, const double* u, double* C) {
+    int n = or_num_unknowns(nl);
+    load_out o = {0};
+    o.n = n; o.C = C;
+    mna_load(nl, p, u, 0.0, CS_MODE_TRAN, &o);
+    return CS_OK;
+}
+
 int or_charges(const cs_netlist* nl, const double* p, const double* u,
                double* q, int* is_diff) {
     int n = or_num_unknowns(nl);
-    load_out o = {n, NULL, q, NULL, NULL};
+    load_out o = {0};
+    o.n = n; o.Q = q;
     mna_load(nl, p, u, 0.0, CS_MODE_TRAN, &o);
     if (is_diff) {
         memset(is_diff, 0, sizeof(int) * n);
@@ -432,6 +473,160 @@ void or_lu_solve(int n, const double* LU, const int* piv, double* b) {
 }
 
 /* ------------------------------------------------------------------ */
+/* linear systems: dense (parity checks) or sparse (baseline, large n)  */
+/* ------------------------------------------------------------------ */
+static int g_sparse = 0;
+void or_set_sparse(int on) { g_sparse = on ? 1 : 0; }
+
+typedef struct {
+    int n, sparse;
+    double *G, *C, *J;
+    int* piv;
+    int nnz, *colptr, *rowind, *gmap, *cmap;
+    double *Gs, *Cs, *Jv;
+    int *rp_ptr, *rp_pos, *diagpos;     /* per row: CSC positions; (r,r) */
+    sp_lu lu;
+    int analyzed;
+} linsys;
+
+static int cmp_pair(const void* a, const void* b) {
+    const int* x = (const int*)a;
+    const int* y = (const int*)b;
+    if (x[1] != y[1]) return x[1] - y[1];   /* column major */
+    return x[0] - y[0];
+}
+
+static int csc_find(const linsys* ls, int i, int j) {
+    int lo = ls->colptr[j], hi = ls->colptr[j + 1];
+    while (lo < hi) {
+        int m = (lo + hi) / 2;
+        if (ls->rowind[m] < i) lo = m + 1;
+        else hi = m;
+    }
+    return (lo < ls->colptr[j + 1] && ls->rowind[lo] == i) ? lo : -1;
+}
+
+static int ls_init(linsys* ls, const cs_netlist* nl, const double* p, int n, int sparse) {
+    memset(ls, 0, sizeof(*ls));
+    ls->n = n;
+    ls->sparse = sparse;
+    if (!sparse) {
+        ls->G = (double*)calloc((size_t)n * n, sizeof(double));
+        ls->C = (double*)calloc((size_t)n * n, sizeof(double));
+        ls->J = (double*)calloc((size_t)n * n, sizeof(double));
+        ls->piv = (int*)calloc(n, sizeof(int));
+        return (ls->G && ls->C && ls->J && ls->piv) ? CS_OK : CS_E_OOM;
+    }
+    double* z = (double*)calloc(n, sizeof(double));
+    load_out o = {0};
+    o.n = n;
+    o.record = 1;
+    mna_load(nl, p, z, 0.0, CS_MODE_TRAN, &o);
+    free(z);
+    int np_ = o.ng + o.nc + n;
+    int* pairs = (int*)malloc(sizeof(int) * 2 * np_);
+    int m = 0;
+    for (int q = 0; q < o.ng; q++) { pairs[2 * m] = o.rec_g[2 * q]; pairs[2 * m + 1] = o.rec_g[2 * q + 1]; m++; }
+    for (int q = 0; q < o.nc; q++) { pairs[2 * m] = o.rec_c[2 * q]; pairs[2 * m + 1] = o.rec_c[2 * q + 1]; m++; }
+    qsort(pairs, m, sizeof(int) * 2, cmp_pair);
+    ls->colptr = (int*)calloc(n + 1, sizeof(int));
+    ls->rowind = (int*)malloc(sizeof(int) * (m > 0 ? m : 1));
+    int nnz = 0;
+    for (int q = 0; q < m; q++) {
+        if (q > 0 && pairs[2 * q] == pairs[2 * q - 2] && pairs[2 * q + 1] == pairs[2 * q - 1]) continue;
+        ls->rowind[nnz] = pairs[2 * q];
+        ls->colptr[pairs[2 * q + 1] + 1]++;
+        nnz++;
+    }
+    for (int j = 0; j < n; j++) ls->colptr[j + 1] += ls->colptr[j];
+    ls->nnz = nnz;
+    free(pairs);
+    ls->gmap = (int*)malloc(sizeof(int) * (o.ng > 0 ? o.ng : 1));
+    ls->cmap = (int*)malloc(sizeof(int) * (o.nc > 0 ? o.nc : 1));
+    for (int q = 0; q < o.ng; q++) ls->gmap[q] = csc_find(ls, o.rec_g[2 * q], o.rec_g[2 * q + 1]);
+    for (int q = 0; q < o.nc; q++) ls->cmap[q] = csc_find(ls, o.rec_c[2 * q], o.rec_c[2 * q + 1]);
+    free(o.rec_g);
+    free(o.rec_c);
+    ls->Gs = (double*)calloc(nnz + 1, sizeof(double));
+    ls->Cs = (double*)calloc(nnz + 1, sizeof(double));
+    ls->Jv = (double*)calloc(nnz + 1, sizeof(double));
+    ls->rp_ptr = (int*)calloc(n + 1, sizeof(int));
+    ls->rp_pos = (int*)malloc(sizeof(int) * (nnz + 1));
+    ls->diagpos = (int*)malloc(sizeof(int) * n);
+    for (int e = 0; e < nnz; e++) ls->rp_ptr[ls->rowind[e] + 1]++;
+    for (int i = 0; i < n; i++) ls->rp_ptr[i + 1] += ls->rp_ptr[i];
+    int* fill = (int*)calloc(n, sizeof(int));
+    for (int j = 0; j < n; j++)
+        for (int e = ls->colptr[j]; e < ls->colptr[j + 1]; e++) {
+            int i = ls->rowind[e];
+            ls->rp_pos[ls->rp_ptr[i] + fill[i]++] = e;
+        }
+    free(fill);
+    for (int i = 0; i < n; i++) ls->diagpos[i] = csc_find(ls, i, i);
+    return CS_OK;
+}
+
+static void ls_free(linsys* ls) {
+    free(ls->G); free(ls->C); free(ls->J); free(ls->piv);
+    free(ls->colptr); free(ls->rowind); free(ls->gmap); free(ls->cmap);
+    free(ls->Gs); free(ls->Cs); free(ls->Jv); free(ls->rp_ptr); free(ls->rp_pos);
+    free(ls->diagpos);
+    if (ls->analyzed) sp_free(&ls->lu);
+    memset(ls, 0, sizeof(*ls));
+}
+
+/* F, Q (may be NULL) and the matrices G, C at u */
+static void ls_load(linsys* ls, const cs_netlist* nl, const double* p, const double* u,
+                    double t, int mode, double* F, double* Q) {
+    load_out o = {0};
+    o.n = ls->n;
+    o.F = F;
+    o.Q = Q;
+    if (!ls->sparse) {
+        o.G = ls->G;
+        o.C = ls->C;
+    } else {
+        memset(ls->Gs, 0, sizeof(double) * ls->nnz);
+        memset(ls->Cs, 0, sizeof(double) * ls->nnz);
+        o.Gs = ls->Gs; o.Cs = ls->Cs; o.gmap = ls->gmap; o.cmap = ls->cmap;
+    }
+    mna_load(nl, p, u, t, mode, &o);
+}
+
+/* J = G + a0 C (rows flagged in pin become identity rows), then factor */
+static int ls_factor(linsys* ls, double a0, const int* pin) {
+    const int n = ls->n;
+    if (!ls->sparse) {
+        for (size_t e = 0; e < (size_t)n * n; e++) ls->J[e] = ls->G[e] + a0 * ls->C[e];
+        if (pin)
+            for (int r = 0; r < n; r++)
+                if (pin[r]) {
+                    for (int j = 0; j < n; j++) ls->J[(size_t)r * n + j] = 0.0;
+                    ls->J[(size_t)r * n + r] = 1.0;
+                }
+        return or_lu_factor(n, ls->J, ls->piv);
+    }
+    for (int e = 0; e < ls->nnz; e++) ls->Jv[e] = ls->Gs[e] + a0 * ls->Cs[e];
+    if (pin)
+        for (int r = 0; r < n; r++)
+            if (pin[r]) {
+                if (ls->diagpos[r] < 0) return CS_E_SINGULAR;
+                for (int q = ls->rp_ptr[r]; q < ls->rp_ptr[r + 1]; q++) ls->Jv[ls->rp_pos[q]] = 0.0;
+                ls->Jv[ls->diagpos[r]] = 1.0;
+            }
+    if (!ls->analyzed) {
+        sp_analyze(&ls->lu, n, ls->colptr, ls->rowind);
+        ls->analyzed = 1;
+    }
+    return sp_refactor(&ls->lu, ls->colptr, ls->rowind, ls->Jv) == 0 ? CS_OK : CS_E_SINGULAR;
+}
+
+static void ls_solve(linsys* ls, double* b) {
+    if (!ls->sparse) or_lu_solve(ls->n, ls->J, ls->piv, b);
+    else sp_solve(&ls->lu, b);
+}
+
+/* ------------------------------------------------------------------ */
 /* helpers                                                             */
 /* ------------------------------------------------------------------ */
 static double wrms(int n, const double* x, const double* ua, const double* ub,
@@ -457,30 +652,26 @@ static double maxabs(int n, const double* x) {
 
 typedef struct {
     int n;
-    double *F, *Q, *G, *C, *J, *R, *d;
-    int* piv;
+    double *F, *Q, *R, *d;
     int* is_diff;
+    linsys ls;
 } work_t;
 
-static int work_alloc(work_t* w, int n) {
+static int work_alloc(work_t* w, const cs_netlist* nl, const double* p, int n) {
     memset(w, 0, sizeof(*w));
     w->n = n;
     w->F = (double*)calloc(n, sizeof(double));
     w->Q = (double*)calloc(n, sizeof(double));
     w->R = (double*)calloc(n, sizeof(double));
     w->d = (double*)calloc(n, sizeof(double));
-    w->G = (double*)calloc((size_t)n * n, sizeof(double));
-    w->C = (double*)calloc((size_t)n * n, sizeof(double));
-    w->J = (double*)calloc((size_t)n * n, sizeof(double));
-    w->piv = (int*)calloc(n, sizeof(int));
     w->is_diff = (int*)calloc(n, sizeof(int));
-    return (w->F && w->Q && w->R && w->d && w->G && w->C && w->J && w->piv &&
-            w->is_diff) ? CS_OK : CS_E_OOM;
+    if (!(w->F && w->Q && w->R && w->d && w->is_diff)) return CS_E_OOM;
+    return ls_init(&w->ls, nl, p, n, g_sparse);
 }
 
 static void work_free(work_t* w) {
-    free(w->F); free(w->Q); free(w->R); free(w->d);
-    free(w->G); free(w->C); free(w->J); free(w->piv); free(w->is_diff);
+    free(w->F); free(w->Q); free(w->R); free(w->d); free(w->is_diff);
+    ls_free(&w->ls);
 }
 
 /* DC operating point: Newton on F(u) = 0 in DC mode (charges drop out). */
@@ -489,15 +680,13 @@ static int dcop_core(const cs_netlist* nl, const double* p, double t0,
                      cs_tran_stats* st) {
     int n = w->n;
     for (int it = 0; it < maxit; it++) {
-        load_out o = {n, w->F, NULL, w->G, NULL};
-        mna_load(nl, p, u, t0, CS_MODE_DCOP, &o);
+        ls_load(&w->ls, nl, p, u, t0, CS_MODE_DCOP, w->F, NULL);
         double fn = maxabs(n, w->F);
         if (!isfinite(fn)) return CS_E_NOCONV;
         if (fn <= tol) return CS_OK;
-        memcpy(w->J, w->G, sizeof(double) * n * n);
-        if (or_lu_factor(n, w->J, w->piv) != CS_OK) return CS_E_SINGULAR;
+        if (ls_factor(&w->ls, 0.0, NULL) != CS_OK) return CS_E_SINGULAR;
         for (int i = 0; i < n; i++) w->d[i] = -w->F[i];
-        or_lu_solve(n, w->J, w->piv, w->d);
+        ls_solve(&w->ls, w->d);
         for (int i = 0; i < n; i++) u[i] += w->d[i];
         if (st) st->newton_iters++;
     }
@@ -508,7 +697,7 @@ int or_dcop(const cs_netlist* nl, const double* p, const cs_tran_opts* o,
             double* u_out, cs_tran_stats* st) {
     int n = or_num_unknowns(nl);
     work_t w;
-    if (work_alloc(&w, n) != CS_OK) { work_free(&w); return CS_E_OOM; }
+    if (work_alloc(&w, nl, p, n) != CS_OK) { work_free(&w); return CS_E_OOM; }
     if (st) memset(st, 0, sizeof(*st));
     memset(u_out, 0, sizeof(double) * n);
     int rc = dcop_core(nl, p, o->t_start, o->dc_abstol, 100, u_out, &w, st);
@@ -526,21 +715,15 @@ static int uic_core(const cs_netlist* nl, const double* p, double t0,
     memcpy(u, u0, sizeof(double) * n);
     or_charges(nl, p, u, w->Q, w->is_diff);
     for (int it = 0; it < 50; it++) {
-        load_out o = {n, w->F, NULL, w->G, NULL};
-        mna_load(nl, p, u, t0, CS_MODE_TRAN, &o);
+        ls_load(&w->ls, nl, p, u, t0, CS_MODE_TRAN, w->F, NULL);
         for (int r = 0; r < n; r++)
-            if (w->is_diff[r]) {
-                w->F[r] = u[r] - u0[r];
-                for (int j = 0; j < n; j++) w->G[(size_t)r * n + j] = 0.0;
-                w->G[(size_t)r * n + r] = 1.0;
-            }
+            if (w->is_diff[r]) w->F[r] = u[r] - u0[r];
         double fn = maxabs(n, w->F);
         if (!isfinite(fn)) return CS_E_NOCONV;
         if (fn <= tol) return CS_OK;
-        memcpy(w->J, w->G, sizeof(double) * n * n);
-        if (or_lu_factor(n, w->J, w->piv) != CS_OK) return CS_E_SINGULAR;
+        if (ls_factor(&w->ls, 0.0, w->is_diff) != CS_OK) return CS_E_SINGULAR;
         for (int i = 0; i < n; i++) w->d[i] = -w->F[i];
-        or_lu_solve(n, w->J, w->piv, w->d);
+        ls_solve(&w->ls, w->d);
         for (int i = 0; i < n; i++) u[i] += w->d[i];
         if (st) st->newton_iters++;
     }
@@ -548,7 +731,7 @@ static int uic_core(const cs_netlist* nl, const double* p, double t0,
 }
 
 /* next source breakpoint after t */
-static double next_breakpoint(const cs_netlist* nl, const double* p, double t) {
+double or_next_breakpoint(const cs_netlist* nl, const double* p, double t) {
     cs_param_layout L;
     or_layout(nl, &L);
     double best = INFINITY;
@@ -572,7 +755,7 @@ int or_tran(const cs_netlist* nl, const double* p, const double* u0,
     cs_tran_stats lst;
     if (!st) st = &lst;
     memset(st, 0, sizeof(*st));
-    if (work_alloc(&w, n) != CS_OK) { work_free(&w); return CS_E_OOM; }
+    if (work_alloc(&w, nl, p, n) != CS_OK) { work_free(&w); return CS_E_OOM; }
     double* U = (double*)calloc((size_t)3 * n, sizeof(double));   /* history */
     double* QH = (double*)calloc((size_t)3 * n, sizeof(double));
     double* u = (double*)calloc(n, sizeof(double));
@@ -580,6 +763,7 @@ int or_tran(const cs_netlist* nl, const double* p, const double* u0,
     double* qc = (double*)calloc(n, sizeof(double));
     double* est = (double*)calloc(n, sizeof(double));
     int rc = CS_OK;
+    double t = o->t_start;
     if (!U || !QH || !u || !up || !qc || !est) { rc = CS_E_OOM; goto done; }
 
     const double span = o->t_stop - o->t_start;
@@ -600,23 +784,19 @@ int or_tran(const cs_netlist* nl, const double* p, const double* u0,
     memcpy(U, u, sizeof(double) * n);
     or_charges(nl, p, u, QH, NULL);
     int npts = 1;
-    double t = o->t_start, h = dt_init;
+    double h = dt_init;
     int kout = 0;
-#define REC(k, src)                                                     \
-    do {                                                                \
-        for (int pi = 0; pi < n_probe; pi++)                            \
-            wave[(size_t)(k) * n_probe + pi] = (src)[probe[pi]];        \
-    } while (0)
     if (wave)
         while (kout < o->n_out && o->t_start + kout * o->dt_out <=
                                       o->t_start + 1e-9 * o->dt_out) {
-            REC(kout, u);
+            for (int pi = 0; pi < n_probe; pi++)
+                wave[(size_t)kout * n_probe + pi] = u[probe[pi]];
             kout++;
         }
 
     while (t < o->t_stop) {
         if (st->steps + st->rejected >= o->max_steps) { rc = CS_E_MAXSTEPS; break; }
-        double tb = next_breakpoint(nl, p, t);
+        double tb = or_next_breakpoint(nl, p, t);
         int src_bp = 1;
         if (!(tb < o->t_stop)) { tb = o->t_stop; src_bp = 0; }
         double hh = h < dt_max ? h : dt_max;
@@ -659,14 +839,12 @@ int or_tran(const cs_netlist* nl, const double* p, const double* u0,
         memcpy(u, up, sizeof(double) * n);
         int conv = 0;
         for (int it = 0; it < o->newton_maxit; it++) {
-            load_out lo = {n, w.F, w.Q, w.G, w.C};
-            mna_load(nl, p, u, tn, CS_MODE_TRAN, &lo);
+            ls_load(&w.ls, nl, p, u, tn, CS_MODE_TRAN, w.F, w.Q);
             for (int i = 0; i < n; i++) w.R[i] = w.F[i] + a[0] * w.Q[i] + qc[i];
-            for (size_t e = 0; e < (size_t)n * n; e++) w.J[e] = w.G[e] + a[0] * w.C[e];
             st->newton_iters++;
-            if (or_lu_factor(n, w.J, w.piv) != CS_OK) break;
+            if (ls_factor(&w.ls, a[0], NULL) != CS_OK) break;
             for (int i = 0; i < n; i++) w.d[i] = -w.R[i];
-            or_lu_solve(n, w.J, w.piv, w.d);
+            ls_solve(&w.ls, w.d);
             for (int i = 0; i < n; i++) u[i] += w.d[i];
             double nr = wrms(n, w.d, u, NULL, rtol, atol);
             if (!(nr == nr)) break;
@@ -736,9 +914,8 @@ int or_tran(const cs_netlist* nl, const double* p, const double* u0,
             if (h < dt_min) h = dt_min;
         }
     }
-#undef REC
-    st->t_end = t;
 done:
+    st->t_end = t;
     st->retcode = rc;
     if (u_final && U) memcpy(u_final, U, sizeof(double) * n);
     free(U); free(QH); free(u); free(up); free(qc); free(est);

@@ -29,6 +29,22 @@ double or_wave_value(int kind, const double* d, int len, double t);
 double or_wave_next_bp(int kind, const double* d, int len, double t);
 double or_pwl_at_time(const double* tsys, int npts, double t);
 
+/* Earliest breakpoint of any source strictly after t (+Inf if none). */
+double or_next_breakpoint(const cs_netlist* nl, const double* p, double t);
+
+/* Discrete adjoint (adjoint.c): energy delivered by voltage source vsrc over
+ * the run and its gradient with respect to every MOSFET width; optional log
+ * of the accepted times and segment counts for replay. */
+int    or_adjoint(const cs_netlist* nl, const double* p, const double* u0,
+                  const cs_tran_opts* o, int vsrc, double* E, double* grad,
+                  cs_tran_stats* st, long long* n_steps, double* t_log,
+                  int* npts_log, long long log_cap);
+/* Energy of a replay of a logged step sequence, Newton converged to ntol. */
+int    or_replay_energy(const cs_netlist* nl, const double* p, const double* u0,
+                        const cs_tran_opts* o, int vsrc, long long M,
+                        const double* t_log, const int* npts_log, double ntol,
+                        double* E);
+
 /* Square-law MOSFET: current d->s and its partials wrt Vd, Vg, Vs. */
 void   or_sqlaw(int type, double vth, double kp, double w, double l, double gmin,
                 double vd, double vg, double vs,
@@ -39,9 +55,16 @@ int    or_residual(const cs_netlist* nl, const double* p, const double* u,
                    const double* du, double t, int mode, double* out);
 int    or_jacobian(const cs_netlist* nl, const double* p, const double* u,
                    double gamma, double t, int mode, double* J);
+/* Capacitance matrix dQ/du (dense row-major), computed directly. */
+int    or_cmatrix(const cs_netlist* nl, const double* p, const double* u, double* C);
 /* Charge vector Q(u) and its pattern flag per row (1 = row carries charge). */
 int    or_charges(const cs_netlist* nl, const double* p, const double* u,
                   double* q, int* is_diff);
+
+/* Linear solver of the Newton loops: 0 = dense partial-pivot LU (default,
+ * used by the parity tests), 1 = sparse left-looking LU (sparse.c, used for
+ * the CPU baseline and large circuits).  Set before running. */
+void   or_set_sparse(int on);
 
 /* Dense LU with partial pivoting; solve.  Returns 0 or CS_E_SINGULAR. */
 int    or_lu_factor(int n, double* A, int* piv);

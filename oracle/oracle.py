@@ -89,6 +89,13 @@ def lib():
         L.or_lu_factor.argtypes = [c_int, _D, POINTER(c_int)]
         L.or_lu_solve.argtypes = [c_int, _D, POINTER(c_int), _D]
         L.or_lu_solve.restype = None
+        L.or_adjoint.argtypes = [N, _D, _D, POINTER(_Opts), c_int, _D, _D, POINTER(Stats),
+                                 POINTER(c_longlong), _D, POINTER(c_int), c_longlong]
+        L.or_replay_energy.argtypes = [N, _D, _D, POINTER(_Opts), c_int, c_longlong, _D,
+                                       POINTER(c_int), c_double, _D]
+        L.or_cmatrix.argtypes = [N, _D, _D, _D]
+        L.or_set_sparse.argtypes = [c_int]
+        L.or_set_sparse.restype = None
         L.or_dcop.argtypes = [N, _D, POINTER(_Opts), _D, POINTER(Stats)]
         L.or_tran.argtypes = [N, _D, _D, POINTER(_Opts), c_int, POINTER(c_int), _D, _D,
                               POINTER(Stats)]
@@ -152,6 +159,45 @@ def opts(o=None, **kw) -> _Opts:
     return r
 
 
+def adjoint(flat, p, u0, vsrc=0, o=None, log_cap=2_000_000, **kw):
+    """Energy of voltage source vsrc and dE/dW for every MOSFET, plus the step
+    log (times, segment counts) for replays."""
+    nl = Netlist(flat)
+    oo = opts(o, **kw)
+    E = c_double()
+    nm = len(flat.arrays["mos_vth"])
+    g = np.zeros(max(nm, 1))
+    st = Stats()
+    M = c_longlong()
+    tl = np.zeros(log_cap)
+    npl = np.zeros(log_cap, np.int32)
+    rc = lib().or_adjoint(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                          _dp(np.ascontiguousarray(u0, np.float64)), byref(oo), vsrc, byref(E),
+                          _dp(g), byref(st), byref(M), _dp(tl),
+                          npl.ctypes.data_as(POINTER(c_int)), log_cap)
+    m = M.value
+    return E.value, g[:nm], st, rc, tl[:m + 1].copy(), npl[:m + 1].copy()
+
+
+def replay_energy(flat, p, u0, t_log, npts_log, vsrc=0, o=None, ntol=1e-9, **kw):
+    nl = Netlist(flat)
+    oo = opts(o, **kw)
+    E = c_double()
+    tl = np.ascontiguousarray(t_log, np.float64)
+    npl = np.ascontiguousarray(npts_log, np.int32)
+    rc = lib().or_replay_energy(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                                _dp(np.ascontiguousarray(u0, np.float64)), byref(oo), vsrc,
+                                len(tl) - 1, _dp(tl), npl.ctypes.data_as(POINTER(c_int)), ntol,
+                                byref(E))
+    return E.value, rc
+
+
+def set_sparse(on: bool):
+    """Select the oracle's linear solver: dense partial-pivot LU (default) or the
+    sparse left-looking LU."""
+    lib().or_set_sparse(1 if on else 0)
+
+
 def wave_value(kind, data, t):
     d = np.ascontiguousarray(data, np.float64)
     return lib().or_wave_value(kind, _dp(d), len(d), t)
@@ -188,6 +234,15 @@ def jacobian(flat, p, u, gamma, t, mode=0):
     lib().or_jacobian(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
                       _dp(np.ascontiguousarray(u, np.float64)), gamma, t, mode, _dp(J))
     return J
+
+
+def cmatrix(flat, p, u):
+    """dQ/du, dense."""
+    nl = Netlist(flat)
+    Cm = np.zeros((nl.n, nl.n))
+    lib().or_cmatrix(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                     _dp(np.ascontiguousarray(u, np.float64)), _dp(Cm))
+    return Cm
 
 
 def lu_solve(A, b):

@@ -1,0 +1,167 @@
+"""Host logic on the CPU: the C-ABI library loads and exports the header's
+entry points, and the host symbolic analysis (static pivots, fill pattern,
+schedules) is right — checked by brute force with a dense LU on the permuted
+Jacobian, no GPU involved."""
+import ctypes
+import os
+import re
+from ctypes import byref
+
+import numpy as np
+import pytest
+
+import cedarhip as ch
+from cedarhip import (C, CurrentSource, Gnd, L, Named, R, SquareLawMOSFET, V, VoltageSource, net,
+                      pulse, vccs, vcvs)
+from cedarhip import lib as Lb
+from cedarhip.workloads import inverter, ring_oscillator
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "cedarhip.h")).read()
+    return sorted(set(re.findall(r"\b(cs_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = Lb.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(Lb.SIGNATURES), set(syms) ^ set(Lb.SIGNATURES)
+    assert lib.cs_abi_version() == 1
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    lib = Lb.load_library()
+    n = ctypes.c_int32(-1)
+    rc = lib.cs_device_count(byref(n))
+    if n.value == 0:
+        ctx = ctypes.c_void_p()
+        assert lib.cs_create(0, byref(ctx)) == Lb.CS_E_NODEVICE
+
+
+def analyze(flat):
+    lib = Lb.load_library()
+    nl = Lb.NetlistStruct(flat)
+    info = Lb.cs_circuit_info()
+    rc = lib.cs_analyze(byref(nl.struct), byref(info), None, None, None, None)
+    assert rc == 0, rc
+    rp = np.zeros(info.n + 1, np.int32)
+    col = np.zeros(info.nnz_lu, np.int32)
+    rn = np.zeros(info.n, np.int32)
+    cn = np.zeros(info.n, np.int32)
+    assert lib.cs_analyze(byref(nl.struct), byref(info), Lb.iptr(rp), Lb.iptr(col), Lb.iptr(rn),
+                          Lb.iptr(cn)) == 0
+    return info, rp, col, rn, cn
+
+
+def dense_lu_nopivot(A):
+    A = A.copy()
+    n = A.shape[0]
+    for k in range(n):
+        assert A[k, k] != 0.0
+        A[k + 1:, k] /= A[k, k]
+        A[k + 1:, k + 1:] -= np.outer(A[k + 1:, k], A[k, k + 1:])
+    return A
+
+
+def circuits():
+    def floating_vsrc():
+        a = Named(net, "a")(); b = Named(net, "b")(); c = Named(net, "c")(); gnd = Named(net, "gnd")()
+        Named(V(1.0), "V1")(a, gnd); Named(R(10.0), "R1")(a, b)
+        Named(V(0.3), "V2")(b, c); Named(R(20.0), "R2")(c, gnd)
+        Named(C(1e-9), "C1")(c, gnd); Gnd()(gnd)
+
+    def ladder():
+        gnd = Named(net, "gnd")()
+        ns = [Named(net, f"n{k}")() for k in range(30)]
+        Named(VoltageSource(tran=pulse(0, 1, 0, 1e-9, 1e-9, 5e-9, 20e-9)), "V")(ns[0], gnd)
+        for k in range(29):
+            Named(R(100.0 + k), f"R{k}")(ns[k], ns[k + 1])
+            Named(C(1e-12), f"C{k}")(ns[k + 1], gnd)
+        Named(L(1e-9), "L1")(ns[29], gnd)
+        Gnd()(gnd)
+
+    def controlled():
+        gnd = Named(net, "0")()
+        n = {k: Named(net, f"n{k}")() for k in range(1, 6)}
+        Named(V(1.0), "V1")(gnd, n[1]); Named(R(1e3), "R1")(n[1], gnd)
+        Named(vcvs(2.0), "E1")(gnd, n[2], n[1], gnd); Named(R(1e3), "R2")(n[2], gnd)
+        Named(vccs(1e-3), "G1")(n[3], gnd, n[2], gnd); Named(R(1e3), "R3")(n[3], gnd)
+        Named(CurrentSource(1e-3), "I1")(n[4], n[5]); Named(R(5.0), "R4")(n[4], gnd)
+        Named(R(7.0), "R5")(n[5], gnd); Gnd()(gnd)
+
+    return [("inverter", inverter()), ("ring11", ring_oscillator(11)),
+            ("ring101", ring_oscillator(101)), ("floating_vsrc", floating_vsrc),
+            ("ladder", ladder), ("controlled", controlled)]
+
+
+@pytest.mark.parametrize("name,fn", circuits())
+def test_static_pivots_and_fill_pattern(name, fn):
+    """Permute the oracle's Jacobian by the analysis' pivot order and factor it
+    densely without pivoting: every pivot is nonzero and every nonzero of L+U
+    lies inside the reported fill pattern."""
+    flat = ch.lower(fn)
+    info, rp, col, rn, cn = analyze(flat)
+    assert sorted(rn) == list(range(info.n)) and sorted(cn) == list(range(info.n))
+    pat = np.zeros((info.n, info.n), bool)
+    for k in range(info.n):
+        pat[k, col[rp[k]:rp[k + 1]]] = True
+        assert np.all(np.diff(col[rp[k]:rp[k + 1]]) > 0)
+    assert np.all(np.diag(pat))
+    rng = np.random.default_rng(11)
+    p = flat.params()
+    for trial in range(3):
+        u = rng.uniform(0.0, 1.8, flat.n)
+        J = O.jacobian(flat, p, u, 1e12 * (trial + 1), 1e-9, 0)
+        M = J[np.ix_(rn, cn)]
+        assert np.all((M != 0) <= pat)
+        LU = dense_lu_nopivot(M)
+        assert np.all((np.abs(LU) > 1e-300) <= pat)
+
+
+def test_ring_schedule_is_shallow():
+    """Singleton peeling takes the supply source's branch row and current
+    column first; multiple minimum degree then eliminates the ring by cyclic
+    reduction, so the LU depth grows like log2(N), not N."""
+    depths = {}
+    for N in (11, 51, 101):
+        info = analyze(ch.lower(ring_oscillator(N)))[0]
+        assert info.n_singletons == 2
+        assert info.nnz_lu <= 6 * info.n
+        depths[N] = (info.lu_levels, info.fwd_levels, info.bwd_levels)
+    assert depths[101][0] <= 3 * np.log2(101) + 4, depths
+    assert depths[101][0] - depths[11][0] <= 3 * (np.log2(101) - np.log2(11)) + 3, depths
+
+
+def test_malformed_netlists_are_rejected():
+    lib = Lb.load_library()
+    flat = ch.lower(ring_oscillator(11))
+    bad = ch.FlatCircuit(flat.n_nodes, dict(flat.arrays), flat.gmin, flat.node_names,
+                         flat.unknown_names, flat.device_names, flat.names)
+    bad.arrays["mos_nodes"] = flat.arrays["mos_nodes"].copy()
+    bad.arrays["mos_nodes"][0] = 999
+    nl = Lb.NetlistStruct(bad)
+    info = Lb.cs_circuit_info()
+    assert lib.cs_analyze(byref(nl.struct), byref(info), None, None, None, None) == Lb.CS_E_INVALID
+
+    def floating():
+        a = Named(net, "a")(); b = Named(net, "b")(); gnd = Named(net, "gnd")()
+        Named(V(1.0), "V1")(a, gnd); Named(C(1e-12), "C1")(a, b); Gnd()(gnd)
+        Named(CurrentSource(1e-3), "I")(b, gnd)
+    # node b only has a capacitor: fine in transient (structure exists)
+    info2 = analyze(ch.lower(floating))[0]
+    assert info2.n == 3
+
+
+def test_unimplemented_device_raises():
+    """test/basic.jl:5-19: using an unimplemented device is an error."""
+    def ERR():
+        vcc = Named(net, "vcc")(); gnd = Named(net, "gnd")()
+        Named(ch.UnimplementedDevice(), "U")(vcc, gnd)
+    with pytest.raises(ch.CedarError):
+        ch.lower(ERR)
