@@ -50,6 +50,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--adjoint-inst", type=int, default=0,
+                    help="also time the adjoint workload (51-stage rings, dE/dW, 0-5 ns) "
+                         "with this many instances")
+    ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--kernel", choices=["auto", "lds", "global"], default="auto",
+                    help="batched kernel: LDS-resident plan (auto when it fits) or global plan")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -57,12 +63,20 @@ def main():
 
     import cedarhip as ch
     from cedarhip import lib as L
-    from cedarhip.workloads import ring_mc
+    from cedarhip.workloads import ring_mc, ring_widths
+
+    if args.adjoint_only:
+        eng = ch.Engine(0)
+        print(json.dumps({"adjoint": adjoint_section(ch, L, ring_widths, args, eng)}), flush=True)
+        eng.close()
+        return
 
     log(f"building workload: {args.n_inst} rings, seed {args.seed}")
     flats, params, u0s, stages = ring_mc(n_inst=args.n_inst, seed=args.seed)
+    kern = {"auto": L.BATCH_KERNEL_AUTO, "lds": L.BATCH_KERNEL_LDS,
+            "global": L.BATCH_KERNEL_GLOBAL}[args.kernel]
     opts = ch.TranOptions(t_start=0.0, t_stop=args.t_stop, reltol=1e-6, abstol=1e-9,
-                          init=L.INIT_UIC, max_order=2, n_out=0, dt_out=0.0)
+                          init=L.INIT_UIC, max_order=2, n_out=0, dt_out=0.0, batch_kernel=kern)
     eng = ch.Engine(0)
     systems, key, inst_sys = [], {}, []
     for f in flats:
@@ -114,6 +128,8 @@ def main():
             "n_inst": args.n_inst, "seed": args.seed,
             "newton_iterations": int(iters), "accepted_steps": int(steps_acc),
             "failed_instances": len(bad),
+            "batch_kernel": "lds" if batch.kernel == L.BATCH_KERNEL_LDS else "global",
+            "launches": batch.n_launches,
         },
         "device_evals_per_s": n_mos_tot / (kernel_ms * 1e-3),
         "roofline": {
@@ -123,7 +139,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
-            "kernel": "k_tran_batch",
+            "kernel": "k_tran_lds" if batch.kernel == L.BATCH_KERNEL_LDS else "k_tran_batch",
             "kernel_ms": kernel_ms,
             "alg_bytes_per_launch": res.alg_bytes,
             "alg_flops_per_launch": res.alg_flops,
@@ -162,8 +178,38 @@ def main():
                                          f"0-{args.t_stop * 1e9:g} ns, OpenMP across instances, "
                                          f"sparse left-looking LU (oracle/sparse.c)"}
         log(f"cpu: {k / dt:.3f} inst/s ({dt:.1f} s, {thr} threads, rc {rc})")
+    if args.adjoint_inst > 0:
+        out["adjoint"] = adjoint_section(ch, L, ring_widths, args, eng)
     eng.close()
     print(json.dumps(out), flush=True)
+
+
+def adjoint_section(ch, L, ring_widths, args, eng):
+    """Adjoint workload: n copies of a 51-stage ring, W ~ U[0.5, 4] um for all
+    102 transistors, objective = supply energy over 0-5 ns, gradient w.r.t.
+    every width, checkpoints every 64 accepted steps.  Timed: forward
+    (recording) + reverse kernels, inputs uploaded inside the call."""
+    n = args.adjoint_inst or 1024
+    flats, params, u0s = ring_widths(n_inst=n, nstage=51, seed=args.seed)
+    o = ch.TranOptions(t_start=0.0, t_stop=5e-9, reltol=1e-6, abstol=1e-9, newton_tol=1e-6,
+                       newton_maxit=30, init=L.INIT_UIC)
+    log(f"adjoint: {n} x 51-stage rings, 0-5 ns")
+    t0 = time.perf_counter()
+    try:
+        res = ch.adjoint_batch(flats, params, u0s, o, vsrc=0, ckpt_every=64, max_steps=300_000,
+                               engine=eng)
+    except ch.CedarHIPError as e:
+        log(f"adjoint unavailable: {e}")
+        return {"error": str(e)}
+    dt = time.perf_counter() - t0
+    ok = sum(1 for r in res.retcodes if r == "Success")
+    log(f"adjoint: {dt:.2f} s wall, kernels {res.kernel_ms:.1f} ms, {ok}/{n} ok, "
+        f"mean steps {float(res.n_steps.mean()):.0f}")
+    return {"value": n / (res.kernel_ms * 1e-3), "unit": "gradient instances/s",
+            "kernel_ms": res.kernel_ms, "wall_s": dt, "n_inst": n, "ok": ok,
+            "mean_steps": float(res.n_steps.mean()),
+            "workload": "51-stage ring, W~U[0.5,4]um (102 widths), supply energy 0-5 ns, "
+                        "dE/dW by discrete adjoint, checkpoint every 64 steps"}
 
 
 if __name__ == "__main__":

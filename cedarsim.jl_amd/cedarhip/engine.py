@@ -55,7 +55,7 @@ class TranOptions:
     newton_maxit: int = 8
     init: int = L.INIT_DCOP
     max_steps: int = 10_000_000
-    lanes: int = 0
+    batch_kernel: int = 0
     record_steps: int = 0
 
     def to_c(self) -> L.cs_tran_opts:
@@ -304,6 +304,9 @@ class Batch:
         engine.check(engine.lib.cs_batch_create(engine.ctx, len(cids), L.iptr(cids), L.dptr(self.params),
                                                 L.dptr(self.u0), byref(self._o), byref(bid)))
         self.bid = bid.value
+        kern, nl = L.c_int32(), L.c_int32()
+        engine.check(engine.lib.cs_batch_kernel(engine.ctx, self.bid, byref(kern), byref(nl)))
+        self.kernel, self.n_launches = kern.value, nl.value
         self.n_inst = len(cids)
 
     def run(self) -> float:
@@ -366,6 +369,68 @@ def tran_batch(circuits: Sequence, params: Sequence[np.ndarray], opts: TranOptio
         return b.results(ms)
     finally:
         b.close()
+
+
+class AdjointResult:
+    def __init__(self, energy, grads, n_steps, t_logs, npts_logs, stats, kernel_ms):
+        self.energy = energy          # (n_inst,)
+        self.grads = grads            # list of (n_mos,) arrays: dE/dW per MOSFET
+        self.n_steps = n_steps        # accepted steps per instance
+        self.t_logs = t_logs          # per instance (n_steps+1,) accepted times, or None
+        self.npts_logs = npts_logs    # per instance segment counts, or None
+        self.stats = stats
+        self.kernel_ms = kernel_ms
+
+    @property
+    def retcodes(self):
+        return [L.RETCODE_NAMES.get(s.retcode, str(s.retcode)) for s in self.stats]
+
+
+def adjoint_batch(circuits: Sequence, params: Sequence[np.ndarray], u0: Sequence[np.ndarray],
+                  opts: TranOptions, vsrc: int = 0, ckpt_every: int = 64,
+                  max_steps: int = 400_000, keep_logs: bool = False,
+                  engine: Engine = None) -> AdjointResult:
+    """Supply energy of voltage source ``vsrc`` and its gradient with respect to
+    every MOSFET width, for many instances in one launch (discrete adjoint with
+    checkpoints every ``ckpt_every`` accepted steps).  The role of
+    ODEForwardSensitivityProblem in the reference (test/sensitivity.jl:29-41)."""
+    eng = engine or default_engine()
+    systems: Dict[int, CircuitSystem] = {}
+    cids, nmos = [], []
+    for c in circuits:
+        if id(c) not in systems:
+            systems[id(c)] = eng.load(c)
+        s = systems[id(c)]
+        cids.append(s.cid)
+        nmos.append(len(s.flat.arrays["mos_vth"]))
+    n_inst = len(cids)
+    cid_a = np.array(cids, np.int32)
+    p_all = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64) for p in params]))
+    u_all = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in u0]))
+    o = TranOptions(**{f.name: getattr(opts, f.name) for f in fields(opts)})
+    o.init = L.INIT_UIC
+    oc = o.to_c()
+    ao = L.cs_adjoint_opts(int(vsrc), int(ckpt_every), int(max_steps))
+    E = np.zeros(n_inst)
+    g = np.zeros(max(sum(nmos), 1))
+    ns = np.zeros(n_inst, np.int64)
+    tl = np.zeros(n_inst * max_steps) if keep_logs else None
+    npl = np.zeros(n_inst * max_steps, np.int32) if keep_logs else None
+    st = (L.cs_tran_stats * n_inst)()
+    ms = c_double()
+    eng.check(eng.lib.cs_adjoint_batch(eng.ctx, n_inst, L.iptr(cid_a), L.dptr(p_all), L.dptr(u_all),
+                                       byref(oc), byref(ao), L.dptr(E), L.dptr(g),
+                                       ns.ctypes.data_as(L.POINTER(L.c_int64)), L.dptr(tl),
+                                       L.iptr(npl), st, byref(ms)))
+    grads, go = [], 0
+    for k in nmos:
+        grads.append(g[go:go + k].copy())
+        go += k
+    t_logs = npts_logs = None
+    if keep_logs:
+        t_logs = [tl[i * max_steps:i * max_steps + ns[i] + 1].copy() for i in range(n_inst)]
+        npts_logs = [npl[i * max_steps:i * max_steps + ns[i] + 1].copy() for i in range(n_inst)]
+    return AdjointResult(E, grads, ns, t_logs, npts_logs, list(st), ms.value)
 
 
 class CircuitSweep:

@@ -35,6 +35,7 @@ RETCODE_NAMES = {
 
 MODE_TRAN, MODE_DCOP = 0, 1
 INIT_DCOP, INIT_UIC = 0, 1
+BATCH_KERNEL_AUTO, BATCH_KERNEL_LDS, BATCH_KERNEL_GLOBAL = 0, 1, 2
 
 _I = POINTER(c_int32)
 _D = POINTER(c_double)
@@ -78,12 +79,16 @@ class cs_tran_opts(Structure):
                 ("dt_max", c_double), ("newton_tol", c_double), ("dt_out", c_double),
                 ("dc_abstol", c_double), ("n_out", c_int32), ("max_order", c_int32),
                 ("newton_maxit", c_int32), ("init", c_int32), ("max_steps", c_int64),
-                ("lanes", c_int32), ("record_steps", c_int32)]
+                ("batch_kernel", c_int32), ("record_steps", c_int32)]
 
 
 class cs_tran_stats(Structure):
     _fields_ = [("steps", c_int64), ("rejected", c_int64), ("newton_iters", c_int64),
                 ("t_end", c_double), ("retcode", c_int32), ("pad", c_int32)]
+
+
+class cs_adjoint_opts(Structure):
+    _fields_ = [("vsrc", c_int32), ("ckpt_every", c_int32), ("max_steps", c_int64)]
 
 
 # every entry point declared in include/cedarhip.h: name -> (restype, argtypes)
@@ -114,6 +119,10 @@ SIGNATURES = {
     "cs_batch_results": (c_int32, [c_void_p, c_int32, _D, _D, POINTER(cs_tran_stats)]),
     "cs_batch_destroy": (c_int32, [c_void_p, c_int32]),
     "cs_batch_traffic": (c_int32, [c_void_p, c_int32, _D, _D]),
+    "cs_batch_kernel": (c_int32, [c_void_p, c_int32, _I, _I]),
+    "cs_adjoint_batch": (c_int32, [c_void_p, c_int32, _I, _D, _D, POINTER(cs_tran_opts),
+                                   POINTER(cs_adjoint_opts), _D, _D, POINTER(c_int64), _D, _I,
+                                   POINTER(cs_tran_stats), _D]),
 }
 
 _lib = None

@@ -73,6 +73,24 @@ def ring_initial_state(flat: FlatCircuit, rng: np.random.Generator, noise=1e-3) 
     return u0
 
 
+def ring_widths(n_inst=1024, nstage=51, seed=0, wmin=0.5e-6, wmax=4e-6):
+    """Config: adjoint sensitivity batch.  n_inst copies of an nstage ring with
+    every transistor width W ~ U[wmin, wmax]; seeded perturbed initial state.
+    Returns (flat per instance, params per instance, u0 per instance)."""
+    rng = np.random.default_rng(seed)
+    f = lower(ring_oscillator(nstage))
+    lay = f.layout()
+    nm = len(f.arrays["mos_w"])
+    flats, params, u0s = [], [], []
+    for _ in range(n_inst):
+        p = f.params().copy()
+        p[lay["mos_w"]:lay["mos_w"] + nm] = rng.uniform(wmin, wmax, nm)
+        flats.append(f)
+        params.append(p)
+        u0s.append(ring_initial_state(f, rng))
+    return flats, params, u0s
+
+
 def ring_mc(n_inst=4096, seed=0, nmin=11, nmax=101, sigma_vth=0.030
             ) -> Tuple[List[FlatCircuit], List[np.ndarray], List[np.ndarray], np.ndarray]:
     """Config: batched Monte Carlo of uneven ring oscillators.

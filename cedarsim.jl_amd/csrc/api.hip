@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/cedarhip.h"
+#include "compact.h"
 #include "kernels.h"
 #include "plan.h"
 #include "rules.h"
@@ -38,6 +40,22 @@ struct Circuit {
     double* h_norm = nullptr;   // pinned, NRM_SLOTS
     hipGraphExec_t iter_exec = nullptr;
     hipGraph_t iter_graph = nullptr;
+    // 16-bit LDS plan image (compact.h) for the LDS-resident batched kernel
+    bool compact_ok = false;
+    std::string compact_why;
+    CPlan hcp{};                // device pointers, host copy
+    void* cbuf = nullptr;
+};
+
+constexpr size_t kLdsMaxBytes = 64 * 1024;   // dynamic LDS of one workgroup
+constexpr size_t kSideStreams = 4;           // hardware queues of one process
+
+// One launch of the LDS-resident kernel: the instances of circuits whose
+// LDS footprints are within a factor of each other.
+struct LdsClass {
+    int32_t nr = 1, nblocks = 0, n_inst = 0;
+    size_t lds = 0;
+    LdsArgs args{};
 };
 
 struct Batch {
@@ -52,6 +70,10 @@ struct Batch {
     size_t wave_len = 0, u_len = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    int32_t kernel = CS_BATCH_KERNEL_GLOBAL;   // which kernel runs this batch
+    std::vector<LdsClass> classes;             // LDS-resident kernel: one launch each
+    std::vector<hipEvent_t> class_ev;
+    int32_t* d_queues = nullptr;
 };
 
 }  // namespace
@@ -59,6 +81,7 @@ struct Batch {
 struct cs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;   // default-priority streams for concurrent class launches
     int n_cu = 0;
     std::string err;
     std::vector<std::unique_ptr<Circuit>> circuits;
@@ -96,6 +119,33 @@ struct Packer {
         return off;
     }
 };
+
+// The LDS plan image and its pivot-order side arrays in one allocation (the
+// image first, 16-byte aligned).  A circuit whose indices do not fit 16 bits
+// keeps compact_ok = false and runs on the global-plan kernel.
+int upload_compact(cs_ctx* ctx, Circuit& C) {
+    std::vector<uint16_t> blob;
+    std::vector<int32_t> colnat, diffrow, probe, dev;
+    const char* why = "";
+    CPlan cp{};
+    if (C.cbuf) { (void)hipFree(C.cbuf); C.cbuf = nullptr; }
+    C.compact_ok = build_compact(C.plan, blob, cp, colnat, diffrow, probe, dev, &why);
+    C.compact_why = why;
+    if (!C.compact_ok) return CS_OK;
+    Packer pk;
+    size_t o_blob = pk.add(blob), o_cn = pk.add(colnat), o_df = pk.add(diffrow), o_pr = pk.add(probe),
+           o_dev = pk.add(dev);
+    HIPCHK(ctx, hipMalloc(&C.cbuf, pk.bytes.size()));
+    HIPCHK(ctx, hipMemcpy(C.cbuf, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
+    char* b = (char*)C.cbuf;
+    cp.blob = (const uint16_t*)(b + o_blob);
+    cp.colnat = (const int32_t*)(b + o_cn);
+    cp.diffrow = (const int32_t*)(b + o_df);
+    cp.probe = (const int32_t*)(b + o_pr);
+    cp.dev = (const int32_t*)(b + o_dev);
+    C.hcp = cp;
+    return CS_OK;
+}
 
 int upload_plan(cs_ctx* ctx, Circuit& C) {
     const Plan& P = C.plan;
@@ -146,7 +196,7 @@ int upload_plan(cs_ctx* ctx, Circuit& C) {
     d.is_diff = I(o_isd); d.probe_slot = I(o_ps);
     if (!C.d_dplan) HIPCHK(ctx, hipMalloc(&C.d_dplan, sizeof(DPlan)));
     HIPCHK(ctx, hipMemcpy(C.d_dplan, &C.hd, sizeof(DPlan), hipMemcpyHostToDevice));
-    return CS_OK;
+    return upload_compact(ctx, C);
 }
 
 // ---- single-instance workspace ---------------------------------------------------
@@ -278,6 +328,7 @@ int cs_destroy(cs_ctx* ctx) {
             for (void* p : b->allocs) (void)hipFree(p);
             if (b->ev0) (void)hipEventDestroy(b->ev0);
             if (b->ev1) (void)hipEventDestroy(b->ev1);
+            for (hipEvent_t e : b->class_ev) (void)hipEventDestroy(e);
         }
     for (auto& c : ctx->circuits)
         if (c) {
@@ -287,7 +338,9 @@ int cs_destroy(cs_ctx* ctx) {
             if (c->d_dplan) (void)hipFree(c->d_dplan);
             if (c->wbuf) (void)hipFree(c->wbuf);
             if (c->h_norm) (void)hipHostFree(c->h_norm);
+            if (c->cbuf) (void)hipFree(c->cbuf);
         }
+    for (hipStream_t st : ctx->side) (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CS_OK;
@@ -509,7 +562,7 @@ void cs_tran_opts_default(cs_tran_opts* o) {
     o->newton_maxit = 8;
     o->init = CS_INIT_DCOP;
     o->max_steps = 10000000;
-    o->lanes = 0;
+    o->batch_kernel = 0;
     o->record_steps = 0;
 }
 
@@ -697,6 +750,91 @@ int cs_tran(cs_ctx* ctx, int32_t cid, const double* p, const double* u0, const c
     return rc;
 }
 
+// Size classes of the LDS-resident kernel.  Circuits are sorted by LDS
+// footprint; a class takes circuits down to 3/4 of its largest footprint, so
+// small circuits are not launched with the largest circuit's LDS reservation.
+// Each class is one persistent launch with its own largest-first queue.
+static int build_lds_classes(cs_ctx* ctx, Batch& B, const std::vector<int32_t>& used,
+                             const std::vector<int32_t>& c_nr, const std::vector<size_t>& c_need,
+                             const std::function<int(void**, size_t)>& dalloc) {
+    std::vector<int32_t> cs = used;
+    std::stable_sort(cs.begin(), cs.end(), [&](int32_t a, int32_t b) { return c_need[a] > c_need[b]; });
+    std::vector<int32_t> cls_of(ctx->circuits.size(), -1);
+    std::vector<std::vector<int32_t>> members;
+    for (size_t q = 0; q < cs.size();) {
+        const size_t cap = c_need[cs[q]];
+        std::vector<int32_t> m;
+        while (q < cs.size() && 4 * c_need[cs[q]] >= 3 * cap) m.push_back(cs[q++]);
+        for (int32_t c : m) cls_of[c] = (int32_t)members.size();
+        members.push_back(m);
+    }
+    std::vector<CPlan> cplans(ctx->circuits.size());
+    for (size_t c = 0; c < ctx->circuits.size(); c++)
+        if (ctx->circuits[c] && ctx->circuits[c]->compact_ok) cplans[c] = ctx->circuits[c]->hcp;
+    void *d_cp = nullptr, *d_q = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_cp, sizeof(CPlan) * cplans.size())) ||
+        (rc = dalloc(&d_q, sizeof(int32_t) * members.size())))
+        return rc;
+    HIPCHK(ctx, hipMemcpy(d_cp, cplans.data(), sizeof(CPlan) * cplans.size(), hipMemcpyHostToDevice));
+    B.d_queues = (int32_t*)d_q;
+    for (size_t k = 0; k < members.size(); k++) {
+        LdsClass L;
+        int32_t dbl_max = 2, words_max = 8;
+        for (int32_t c : members[k]) {
+            const CPlan& cp = ctx->circuits[c]->hcp;
+            dbl_max = std::max(dbl_max, (cp.dbl + 1) & ~1);
+            words_max = std::max(words_max, cp.words);
+            L.nr = std::max(L.nr, c_nr[c]);
+        }
+        std::vector<int32_t> order;
+        for (int32_t i = 0; i < B.n_inst; i++)
+            if (cls_of[B.cid[i]] == (int32_t)k) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return c_need[B.cid[a]] > c_need[B.cid[b]];
+        });
+        void* d_order = nullptr;
+        if ((rc = dalloc(&d_order, sizeof(int32_t) * order.size()))) return rc;
+        HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
+        L.n_inst = (int32_t)order.size();
+        L.lds = lds_kernel_bytes(dbl_max, words_max);
+        const int per_cu = lds_max_blocks(L.nr, L.lds);
+        if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "LDS-resident kernel does not fit on a CU");
+        L.nblocks = (int32_t)std::min<int64_t>(L.n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+        LdsArgs& a = L.args;
+        const BatchArgs& g = B.args;
+        a.cplans = (const CPlan*)d_cp;
+        a.plans = g.plans;
+        a.inst_cid = g.inst_cid;
+        a.p_off = g.p_off;
+        a.params = g.params;
+        a.u_off = g.u_off;
+        a.u0 = g.u0;
+        a.w_off = g.w_off;
+        a.wave = g.wave;
+        a.u_final = g.u_final;
+        a.stats = g.stats;
+        a.order = (const int32_t*)d_order;
+        a.queue = B.d_queues + k;
+        a.n_inst = L.n_inst;
+        a.dbl_max = dbl_max;
+        a.o = g.o;
+        B.classes.push_back(L);
+    }
+    while (ctx->side.size() < kSideStreams) {
+        hipStream_t st;
+        HIPCHK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ctx->side.push_back(st);
+    }
+    const size_t nev = std::min(B.classes.size(), kSideStreams);
+    for (size_t k = 0; k < nev; k++) {
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreate(&e));
+        B.class_ev.push_back(e);
+    }
+    return CS_OK;
+}
+
 // ---- batched transient --------------------------------------------------------------
 int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const double* params,
                     const double* u0, const cs_tran_opts* o, int32_t* bid) {
@@ -731,7 +869,44 @@ int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const 
         lin_max = std::max(lin_max, P.n_lin);
         np_max = std::max<int32_t>(np_max, (int32_t)P.probe.size());
     }
-    if (n_max > 64 * 8)
+    // ---- kernel choice: the LDS-resident kernel when every circuit fits ----
+    const int32_t want = o->batch_kernel;
+    if (want < CS_BATCH_KERNEL_AUTO || want > CS_BATCH_KERNEL_GLOBAL)
+        return fail(ctx, CS_E_INVALID, "unknown batch_kernel");
+    std::vector<int32_t> used;
+    {
+        std::vector<char> seen(ctx->circuits.size(), 0);
+        for (int32_t i = 0; i < n_inst; i++)
+            if (!seen[inst_cid[i]]) { seen[inst_cid[i]] = 1; used.push_back(inst_cid[i]); }
+    }
+    std::string lds_why;
+    std::vector<int32_t> c_nr(ctx->circuits.size(), 0);
+    std::vector<size_t> c_need(ctx->circuits.size(), 0);
+    bool lds_fits = true;
+    for (int32_t c : used) {
+        const Circuit& C = *ctx->circuits[c];
+        if (!C.compact_ok) { lds_fits = false; lds_why = C.compact_why; break; }
+        const CPlan& cp = C.hcp;
+        int32_t nr = cp.n <= 64 ? 1 : cp.n <= 128 ? 2 : 4;
+        while (nr < 4 && cp.n_mos > 128 * nr) nr *= 2;
+        if (cp.n > 64 * nr || cp.n_mos > 128 * nr || cp.n_src > 64 * 64) {
+            lds_fits = false;
+            lds_why = "circuit exceeds the LDS-resident kernel's per-lane limits (256 unknowns, 512 devices)";
+            break;
+        }
+        const size_t need = lds_kernel_bytes((cp.dbl + 1) & ~1, cp.words);
+        if (need > kLdsMaxBytes) {
+            lds_fits = false;
+            lds_why = "circuit needs more LDS than one workgroup may hold";
+            break;
+        }
+        c_nr[c] = nr;
+        c_need[c] = need;
+    }
+    if (want == CS_BATCH_KERNEL_LDS && !lds_fits)
+        return fail(ctx, CS_E_RANGE, "LDS-resident batch kernel: " + lds_why);
+    B->kernel = (want != CS_BATCH_KERNEL_GLOBAL && lds_fits) ? CS_BATCH_KERNEL_LDS : CS_BATCH_KERNEL_GLOBAL;
+    if (B->kernel == CS_BATCH_KERNEL_GLOBAL && n_max > 64 * 8)
         return fail(ctx, CS_E_RANGE, "batched path supports up to 512 unknowns per instance; use cs_tran");
     B->nr = n_max <= 64 ? 1 : n_max <= 128 ? 2 : n_max <= 256 ? 4 : 8;
     B->u_len = uo;
@@ -792,18 +967,22 @@ int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const 
     A.lin_max = lin_max;
     A.n_probe_max = np_max;
     A.o = *o;
-    B->lds = batch_lds_bytes(A);
-    int per_cu = batch_max_blocks(B->nr, B->lds);
-    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "batched kernel does not fit: LDS per instance too large");
-    B->nblocks = std::min<int64_t>(n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
-    void *d_g, *d_c, *d_l;
-    if ((rc = dalloc(&d_g, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
-        (rc = dalloc(&d_c, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
-        (rc = dalloc(&d_l, sizeof(double) * (size_t)B->nblocks * lin_max)))
+    if (B->kernel == CS_BATCH_KERNEL_GLOBAL) {
+        B->lds = batch_lds_bytes(A);
+        int per_cu = batch_max_blocks(B->nr, B->lds);
+        if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "batched kernel does not fit: LDS per instance too large");
+        B->nblocks = std::min<int64_t>(n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+        void *d_g, *d_c, *d_l;
+        if ((rc = dalloc(&d_g, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
+            (rc = dalloc(&d_c, sizeof(double) * (size_t)B->nblocks * nnz_max)) ||
+            (rc = dalloc(&d_l, sizeof(double) * (size_t)B->nblocks * lin_max)))
+            return rc;
+        A.glin = (double*)d_g;
+        A.clin = (double*)d_c;
+        A.linbuf = (double*)d_l;
+    } else if ((rc = build_lds_classes(ctx, *B, used, c_nr, c_need, dalloc)) != CS_OK) {
         return rc;
-    A.glin = (double*)d_g;
-    A.clin = (double*)d_c;
-    A.linbuf = (double*)d_l;
+    }
     HIPCHK(ctx, hipEventCreate(&B->ev0));
     HIPCHK(ctx, hipEventCreate(&B->ev1));
     ctx->batches.push_back(std::move(B));
@@ -821,10 +1000,28 @@ int cs_batch_run(cs_ctx* ctx, int32_t bid, double* kernel_ms) {
     if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
-    HIPCHK(ctx, hipMemsetAsync(B->args.queue, 0, sizeof(int32_t), s));
-    HIPCHK(ctx, hipEventRecord(B->ev0, s));
-    HIPCHK(ctx, launch_tran_batch(B->nr, B->args, B->nblocks, B->lds, s));
-    HIPCHK(ctx, hipEventRecord(B->ev1, s));
+    if (B->kernel == CS_BATCH_KERNEL_GLOBAL) {
+        HIPCHK(ctx, hipMemsetAsync(B->args.queue, 0, sizeof(int32_t), s));
+        HIPCHK(ctx, hipEventRecord(B->ev0, s));
+        HIPCHK(ctx, launch_tran_batch(B->nr, B->args, B->nblocks, B->lds, s));
+        HIPCHK(ctx, hipEventRecord(B->ev1, s));
+    } else {
+        // classes largest first, spread over the side streams so that a
+        // class's tail overlaps the next class's start
+        HIPCHK(ctx, hipMemsetAsync(B->d_queues, 0, sizeof(int32_t) * B->classes.size(), s));
+        HIPCHK(ctx, hipEventRecord(B->ev0, s));
+        const size_t ns = B->class_ev.size();
+        for (size_t k = 0; k < ns; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->side[k], B->ev0, 0));
+        for (size_t k = 0; k < B->classes.size(); k++) {
+            const LdsClass& L = B->classes[k];
+            HIPCHK(ctx, launch_tran_lds(L.nr, L.args, L.nblocks, L.lds, ctx->side[k % ns]));
+        }
+        for (size_t k = 0; k < ns; k++) {
+            HIPCHK(ctx, hipEventRecord(B->class_ev[k], ctx->side[k]));
+            HIPCHK(ctx, hipStreamWaitEvent(s, B->class_ev[k], 0));
+        }
+        HIPCHK(ctx, hipEventRecord(B->ev1, s));
+    }
     HIPCHK(ctx, hipEventSynchronize(B->ev1));
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, B->ev0, B->ev1));
@@ -854,7 +1051,16 @@ int cs_batch_destroy(cs_ctx* ctx, int32_t bid) {
     for (void* p : B->allocs) (void)hipFree(p);
     if (B->ev0) (void)hipEventDestroy(B->ev0);
     if (B->ev1) (void)hipEventDestroy(B->ev1);
+    for (hipEvent_t e : B->class_ev) (void)hipEventDestroy(e);
     ctx->batches[bid].reset();
+    return CS_OK;
+}
+
+int cs_batch_kernel(cs_ctx* ctx, int32_t bid, int32_t* kernel, int32_t* n_launches) {
+    Batch* B = get_batch(ctx, bid);
+    if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
+    if (kernel) *kernel = B->kernel;
+    if (n_launches) *n_launches = B->kernel == CS_BATCH_KERNEL_LDS ? (int32_t)B->classes.size() : 1;
     return CS_OK;
 }
 
@@ -882,6 +1088,153 @@ int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes, double* alg_fl
     }
     if (alg_bytes) *alg_bytes = bytes;
     if (alg_flops) *alg_flops = flops;
+    return CS_OK;
+}
+
+// ---- batched adjoint ------------------------------------------------------------------
+int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const double* params,
+                     const double* u0, const cs_tran_opts* o, const cs_adjoint_opts* ao,
+                     double* energy, double* grad, int64_t* n_steps, double* t_log,
+                     int32_t* npts_log, cs_tran_stats* st, double* kernel_ms) {
+    if (!ctx || n_inst <= 0 || !inst_cid || !params || !u0 || !o || !ao || !energy || !grad)
+        return fail(ctx, CS_E_INVALID, "bad argument");
+    if (o->init != CS_INIT_UIC) return fail(ctx, CS_E_INVALID, "the adjoint needs CS_INIT_UIC");
+    if (ao->ckpt_every <= 0 || ao->max_steps <= 0) return fail(ctx, CS_E_INVALID, "bad adjoint options");
+#ifndef CEDAR_ENABLE_GLOBAL_PLAN_ADJOINT
+    // The global-plan adjoint kernels (adjoint.hip) are switched off: a GPU
+    // run of them stalled for minutes on a case that finishes in seconds
+    // under a CPU emulation of the same kernel code, and no cause has been
+    // found.  Build with -DCEDAR_ENABLE_GLOBAL_PLAN_ADJOINT to turn them on.
+    return fail(ctx, CS_E_INVALID, "the batched adjoint is switched off in this build");
+#endif
+    (void)hipSetDevice(ctx->device);
+    std::vector<void*> allocs;
+    struct Guard {
+        std::vector<void*>& a;
+        ~Guard() { for (void* p : a) (void)hipFree(p); }
+    } guard{allocs};
+    auto dalloc = [&](void** p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) return fail(ctx, CS_E_OOM, "adjoint allocation failed");
+        allocs.push_back(*p);
+        return CS_OK;
+    };
+    std::vector<int64_t> p_off, u_off, g_off;
+    int64_t po = 0, uo = 0, go = 0;
+    int32_t nnz_max = 1, n_max = 1, st_max = 1, lin_max = 1;
+    std::vector<DPlan> plans(ctx->circuits.size());
+    std::vector<int32_t> obj(3 * ctx->circuits.size(), 0);
+    for (size_t c = 0; c < ctx->circuits.size(); c++)
+        if (ctx->circuits[c]) {
+            plans[c] = ctx->circuits[c]->hd;
+            const Plan& P = ctx->circuits[c]->plan;
+            if (ao->vsrc >= 0 && 2 * ao->vsrc + 1 < (int32_t)P.vsrc_nodes.size()) {
+                obj[3 * c] = P.vsrc_nodes[2 * ao->vsrc];
+                obj[3 * c + 1] = P.vsrc_nodes[2 * ao->vsrc + 1];
+                obj[3 * c + 2] = P.n_nodes + ao->vsrc;
+            }
+        }
+    for (int32_t i = 0; i < n_inst; i++) {
+        Circuit* C = get_circuit(ctx, inst_cid[i]);
+        if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id");
+        const Plan& P = C->plan;
+        if (!P.uic_ok) return fail(ctx, CS_E_INVALID, "UIC not supported by this circuit");
+        if (2 * ao->vsrc + 1 >= (int32_t)P.vsrc_nodes.size())
+            return fail(ctx, CS_E_INVALID, "objective source index out of range");
+        p_off.push_back(po); u_off.push_back(uo); g_off.push_back(go);
+        po += P.lay.n_params; uo += P.n; go += P.n_mos;
+        nnz_max = std::max(nnz_max, P.nnz); n_max = std::max(n_max, P.n);
+        st_max = std::max(st_max, P.n_stamp); lin_max = std::max(lin_max, P.n_lin);
+    }
+    if (n_max > 512) return fail(ctx, CS_E_RANGE, "adjoint supports up to 512 unknowns per instance");
+    const int nr = n_max <= 64 ? 1 : n_max <= 128 ? 2 : n_max <= 256 ? 4 : 8;
+    std::vector<int32_t> order(n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return ctx->circuits[inst_cid[a]]->plan.nnz > ctx->circuits[inst_cid[b]]->plan.nnz;
+    });
+    AdjArgs A{};
+    BatchArgs& B = A.b;
+    const int64_t cap = ao->max_steps;
+    const int64_t ckcap = cap / ao->ckpt_every + 1;
+    int rc;
+    void *d_plans, *d_cid, *d_poff, *d_params, *d_uoff, *d_u0, *d_uf, *d_st, *d_order, *d_q, *d_q2,
+        *d_tl, *d_npl, *d_ck, *d_ns, *d_E, *d_g, *d_goff, *d_obj;
+    if ((rc = dalloc(&d_plans, sizeof(DPlan) * plans.size())) ||
+        (rc = dalloc(&d_cid, sizeof(int32_t) * n_inst)) ||
+        (rc = dalloc(&d_poff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_params, sizeof(double) * po)) ||
+        (rc = dalloc(&d_uoff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_u0, sizeof(double) * uo)) || (rc = dalloc(&d_uf, sizeof(double) * uo)) ||
+        (rc = dalloc(&d_st, sizeof(cs_tran_stats) * n_inst)) ||
+        (rc = dalloc(&d_order, sizeof(int32_t) * n_inst)) ||
+        (rc = dalloc(&d_q, sizeof(int32_t))) || (rc = dalloc(&d_q2, sizeof(int32_t))) ||
+        (rc = dalloc(&d_tl, sizeof(double) * cap * n_inst)) ||
+        (rc = dalloc(&d_npl, sizeof(int32_t) * cap * n_inst)) ||
+        (rc = dalloc(&d_ck, sizeof(double) * (size_t)ckcap * 3 * n_max * n_inst)) ||
+        (rc = dalloc(&d_ns, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_E, sizeof(double) * n_inst)) || (rc = dalloc(&d_g, sizeof(double) * go)) ||
+        (rc = dalloc(&d_goff, sizeof(int64_t) * n_inst)) ||
+        (rc = dalloc(&d_obj, sizeof(int32_t) * obj.size())))
+        return rc;
+    HIPCHK(ctx, hipMemcpy(d_plans, plans.data(), sizeof(DPlan) * plans.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_cid, inst_cid, sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_poff, p_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_params, params, sizeof(double) * po, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_uoff, u_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_u0, u0, sizeof(double) * uo, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_goff, g_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_obj, obj.data(), sizeof(int32_t) * obj.size(), hipMemcpyHostToDevice));
+    B.plans = (const DPlan*)d_plans; B.inst_cid = (const int32_t*)d_cid;
+    B.p_off = (const int64_t*)d_poff; B.params = (const double*)d_params;
+    B.u_off = (const int64_t*)d_uoff; B.u0 = (const double*)d_u0; B.w_off = nullptr;
+    B.wave = nullptr; B.u_final = (double*)d_uf; B.stats = (cs_tran_stats*)d_st;
+    B.order = (const int32_t*)d_order; B.queue = (int32_t*)d_q;
+    B.n_inst = n_inst; B.nnz_max = nnz_max; B.n_max = n_max; B.stamp_max = st_max;
+    B.lin_max = lin_max; B.n_probe_max = 0; B.o = *o;
+    B.o.n_out = 0;
+    A.t_log = (double*)d_tl; A.npts_log = (int32_t*)d_npl; A.cap = cap;
+    A.ckpt = (double*)d_ck; A.ckcap = ckcap; A.K = ao->ckpt_every;
+    A.n_steps = (int64_t*)d_ns; A.energy = (double*)d_E; A.grad = (double*)d_g;
+    A.g_off = (const int64_t*)d_goff; A.obj = (const int32_t*)d_obj; A.queue2 = (int32_t*)d_q2;
+    const size_t lds = adj_lds_bytes(B);
+    const int per_cu = adj_max_blocks(nr, lds);
+    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "adjoint kernels do not fit");
+    const int nblocks = (int)std::min<int64_t>(n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+    void *d_gl, *d_cl, *d_lb, *d_seg;
+    if ((rc = dalloc(&d_gl, sizeof(double) * (size_t)nblocks * nnz_max)) ||
+        (rc = dalloc(&d_cl, sizeof(double) * (size_t)nblocks * nnz_max)) ||
+        (rc = dalloc(&d_lb, sizeof(double) * (size_t)nblocks * lin_max)) ||
+        (rc = dalloc(&d_seg, sizeof(double) * (size_t)nblocks * ao->ckpt_every * n_max)))
+        return rc;
+    B.glin = (double*)d_gl; B.clin = (double*)d_cl; B.linbuf = (double*)d_lb;
+    A.seg = (double*)d_seg;
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    HIPCHK(ctx, hipMemsetAsync(d_q, 0, sizeof(int32_t), s));
+    HIPCHK(ctx, hipMemsetAsync(d_q2, 0, sizeof(int32_t), s));
+    HIPCHK(ctx, hipEventRecord(e0, s));
+    HIPCHK(ctx, launch_adj(nr, false, A, nblocks, lds, s));
+    HIPCHK(ctx, launch_adj(nr, true, A, nblocks, lds, s));
+    HIPCHK(ctx, hipEventRecord(e1, s));
+    HIPCHK(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    HIPCHK(ctx, hipMemcpy(energy, d_E, sizeof(double) * n_inst, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(grad, d_g, sizeof(double) * go, hipMemcpyDeviceToHost));
+    if (n_steps) HIPCHK(ctx, hipMemcpy(n_steps, d_ns, sizeof(int64_t) * n_inst, hipMemcpyDeviceToHost));
+    if (t_log) HIPCHK(ctx, hipMemcpy(t_log, d_tl, sizeof(double) * cap * n_inst, hipMemcpyDeviceToHost));
+    if (npts_log)
+        HIPCHK(ctx, hipMemcpy(npts_log, d_npl, sizeof(int32_t) * cap * n_inst, hipMemcpyDeviceToHost));
+    std::vector<cs_tran_stats> sts(n_inst);
+    HIPCHK(ctx, hipMemcpy(sts.data(), d_st, sizeof(cs_tran_stats) * n_inst, hipMemcpyDeviceToHost));
+    if (st) std::memcpy(st, sts.data(), sizeof(cs_tran_stats) * n_inst);
     return CS_OK;
 }
 

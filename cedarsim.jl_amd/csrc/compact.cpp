@@ -1,0 +1,243 @@
+// compact.cpp — builds the 16-bit LDS plan image (compact.h) from a Plan.
+//
+// Same F pattern and the same entry-level LU tasks as the global plan, with
+//   * the forward substitution merged into the LU levels: y_k is one more
+//     task whose level is one past the latest of its L entries and y inputs;
+//   * MOSFET stamps reduced to (id, dI/dVd, dI/dVg) in structure-of-arrays
+//     order (dI/dVs is their negated sum);
+//   * the linear stamps kept only where an entry has a G or C contribution.
+#include "compact.h"
+
+#include <algorithm>
+
+#include "plan.h"
+
+namespace cedar {
+
+namespace {
+
+struct Blob {
+    std::vector<uint16_t> w;
+    bool ok = true;
+    void align(int a) {
+        while (w.size() % a) w.push_back(0);
+    }
+    int32_t add(const std::vector<int32_t>& v, int a = 1) {
+        align(a);
+        int32_t off = (int32_t)w.size();
+        for (int32_t x : v) {
+            if (x < 0 || x > 0xFFFF) ok = false;
+            w.push_back((uint16_t)(x & 0xFFFF));
+        }
+        return off;
+    }
+};
+
+}  // namespace
+
+bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
+                   std::vector<int32_t>& colnat, std::vector<int32_t>& diffrow,
+                   std::vector<int32_t>& probe, std::vector<int32_t>& dev, const char** why) {
+    const int32_t n = P.n, nnz = P.nnz, nm = P.n_mos, ns = P.n_src;
+    cp = CPlan{};
+    *why = "";
+    if (n >= 0x7FFF || nnz >= 0xFFFF || 3 * nm + ns >= 0x7FFF) {
+        *why = "circuit too large for 16-bit indices";
+        return false;
+    }
+    // stamp index map: old 4-per-device layout -> SoA 3-per-device
+    auto conv = [&](int32_t ref, std::vector<int32_t>& out) {
+        const int32_t s = ref >> 1;
+        const bool neg = ref & 1;
+        if (s < 4 * nm) {
+            const int32_t m = s / 4, j = s % 4;
+            if (j == 0) out.push_back(sref(m, neg));
+            else if (j == 1) out.push_back(sref(nm + m, neg));
+            else if (j == 2) out.push_back(sref(2 * nm + m, neg));
+            else {
+                out.push_back(sref(nm + m, !neg));
+                out.push_back(sref(2 * nm + m, !neg));
+            }
+        } else {
+            out.push_back(sref(3 * nm + (s - 4 * nm), neg));
+        }
+    };
+    std::vector<int32_t> jptr(nnz + 1, 0), jref;
+    for (int32_t e = 0; e < nnz; e++) {
+        for (int32_t q = P.jnl_ptr[e]; q < P.jnl_ptr[e + 1]; q++) conv(P.jnl_ref[q], jref);
+        jptr[e + 1] = (int32_t)jref.size();
+    }
+    std::vector<int32_t> rptr(n + 1, 0), rref;
+    for (int32_t k = 0; k < n; k++) {
+        for (int32_t q = P.rnl_ptr[k]; q < P.rnl_ptr[k + 1]; q++) conv(P.rnl_ref[q], rref);
+        rptr[k + 1] = (int32_t)rref.size();
+    }
+    // linear entries
+    std::vector<int32_t> lptr(n + 1, 0), lpos, lcol;
+    for (int32_t k = 0; k < n; k++) {
+        for (int32_t e = P.rowptr[k]; e < P.rowptr[k + 1]; e++)
+            if (P.g_ptr[e + 1] > P.g_ptr[e] || P.c_ptr[e + 1] > P.c_ptr[e]) {
+                lpos.push_back(e);
+                lcol.push_back(P.colperm[e]);
+            }
+        lptr[k + 1] = (int32_t)lpos.size();
+    }
+    // LU tasks with their levels (recomputed from the task list: level of a
+    // task = 1 + max level of the entries it reads; a diagonal's level gates
+    // the L entries scaled by it)
+    const int32_t ntask = (int32_t)P.lu_task.size();
+    std::vector<int32_t> lev(nnz, -1);
+    std::vector<int32_t> tlev(ntask);
+    for (int32_t l = 0; l + 1 < (int32_t)P.lu_lptr.size(); l++)
+        for (int32_t q = P.lu_lptr[l]; q < P.lu_lptr[l + 1]; q++) {
+            tlev[q] = l;
+            lev[P.lu_task[q].e] = l;
+        }
+    int32_t nA = (int32_t)P.lu_lptr.size() - 1;
+    // forward substitution tasks: y_k for rows with L entries
+    std::vector<int32_t> yl(n, -1);
+    for (int32_t k = 0; k < n; k++) {
+        int32_t mx = -1;
+        bool any = false;
+        for (int32_t e = P.rowptr[k]; e < P.diag[k]; e++) {
+            any = true;
+            mx = std::max(mx, std::max(lev[e], yl[P.colperm[e]]));
+        }
+        if (any) {
+            yl[k] = mx + 1;
+            nA = std::max(nA, yl[k] + 1);
+        }
+    }
+    // backward levels
+    std::vector<int32_t> bl(n, -1);
+    int32_t nB = 0;
+    for (int32_t k = n - 1; k >= 0; k--) {
+        int32_t mx = -1;
+        for (int32_t e = P.diag[k] + 1; e < P.rowptr[k + 1]; e++) mx = std::max(mx, bl[P.colperm[e]]);
+        bl[k] = mx + 1;
+        nB = std::max(nB, bl[k] + 1);
+    }
+    // emit tasks level by level
+    std::vector<int32_t> ftask, xtask, upd, aft(nA + 1, 0), axt(nA + 1, 0), bxt(nB + 1, 0);
+    auto push_upd = [&](int32_t a, int32_t b) {
+        upd.push_back(a);
+        upd.push_back(b);
+    };
+    for (int32_t l = 0; l < nA; l++) {
+        if (l < (int32_t)P.lu_lptr.size() - 1)
+            for (int32_t q = P.lu_lptr[l]; q < P.lu_lptr[l + 1]; q++) {
+                const LuTask& t = P.lu_task[q];
+                const int32_t ub = (int32_t)upd.size() / 2;
+                for (int32_t u = t.ub; u < t.ue; u++) push_upd(P.lu_upd[2 * u], P.lu_upd[2 * u + 1]);
+                const int32_t ue = (int32_t)upd.size() / 2;
+                int32_t ctl;
+                if (t.piv >= 0) ctl = t.piv;
+                else if (t.piv == -1) ctl = (int32_t)CP_PLAIN_U;
+                else ctl = (int32_t)CP_DIAG | (-2 - t.piv);
+                ftask.insert(ftask.end(), {t.e, ub, ue, ctl});
+            }
+        aft[l + 1] = (int32_t)ftask.size() / 4;
+        for (int32_t k = 0; k < n; k++) {
+            if (yl[k] != l) continue;
+            const int32_t ub = (int32_t)upd.size() / 2;
+            for (int32_t e = P.rowptr[k]; e < P.diag[k]; e++) push_upd(e, P.colperm[e]);
+            const int32_t ue = (int32_t)upd.size() / 2;
+            xtask.insert(xtask.end(), {k, ub, ue, 0});
+        }
+        axt[l + 1] = (int32_t)xtask.size() / 4;
+    }
+    bxt[0] = (int32_t)xtask.size() / 4;
+    for (int32_t l = 0; l < nB; l++) {
+        for (int32_t k = 0; k < n; k++) {
+            if (bl[k] != l) continue;
+            const int32_t ub = (int32_t)upd.size() / 2;
+            for (int32_t e = P.diag[k] + 1; e < P.rowptr[k + 1]; e++) push_upd(e, P.colperm[e]);
+            const int32_t ue = (int32_t)upd.size() / 2;
+            xtask.insert(xtask.end(), {k, ub, ue, 1});
+        }
+        bxt[l + 1] = (int32_t)xtask.size() / 4;
+    }
+    if (upd.size() / 2 >= 0xFFFF) {
+        *why = "too many LU updates for 16-bit indices";
+        return false;
+    }
+    // sources: data and dc value copied to wd
+    std::vector<int32_t> src;
+    int32_t nwd = 0;
+    for (int32_t s = 0; s < ns; s++) {
+        src.insert(src.end(), {P.src[s].kind, nwd, P.src[s].len, 0});
+        nwd += P.src[s].len + 1;
+    }
+
+    std::vector<int32_t> rdesc(8 * (size_t)n), adesc(4 * (size_t)nA), bdesc(2 * (size_t)nB);
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t e0 = P.rowptr[k];
+        const int32_t d[8] = {e0, P.rowptr[k + 1], P.diag[k], jptr[e0],
+                              lptr[k], lptr[k + 1], rptr[k], rptr[k + 1]};
+        for (int q = 0; q < 8; q++) rdesc[8 * k + q] = d[q];
+    }
+    for (int32_t l = 0; l < nA; l++) {
+        adesc[4 * l] = aft[l];
+        adesc[4 * l + 1] = aft[l + 1];
+        adesc[4 * l + 2] = axt[l];
+        adesc[4 * l + 3] = axt[l + 1];
+    }
+    for (int32_t l = 0; l < nB; l++) {
+        bdesc[2 * l] = bxt[l];
+        bdesc[2 * l + 1] = bxt[l + 1];
+    }
+    Blob b;
+    cp.o_rdesc = b.add(rdesc, 8);
+    cp.o_jptr = b.add(jptr);
+    cp.o_jref = b.add(jref);
+    cp.o_rref = b.add(rref);
+    cp.o_lpos = b.add(lpos);
+    cp.o_lcol = b.add(lcol);
+    cp.o_adesc = b.add(adesc, 4);
+    cp.o_bdesc = b.add(bdesc, 2);
+    cp.o_src = b.add(src, 4);
+    cp.o_ftask = b.add(ftask, 4);
+    cp.o_xtask = b.add(xtask, 4);
+    cp.o_upd = b.add(upd, 2);
+    b.align(8);
+    if (!b.ok) {
+        *why = "plan index does not fit 16 bits";
+        return false;
+    }
+    blob = std::move(b.w);
+    cp.n = n;
+    cp.nnz = nnz;
+    cp.n_mos = nm;
+    cp.n_src = ns;
+    cp.nlin = (int32_t)lpos.size();
+    cp.stamp_n = 3 * nm + ns;
+    cp.nwd = nwd;
+    cp.nA = nA;
+    cp.nB = nB;
+    cp.words = (int32_t)blob.size();
+    cp.dbl = nnz + 2 * cp.nlin + cp.stamp_n + 3 * n + nwd;
+    cp.n_ftask = (int32_t)ftask.size() / 4;
+    cp.n_xtask = (int32_t)xtask.size() / 4;
+    cp.n_upd = (int32_t)upd.size() / 2;
+
+    // pivot-order side arrays
+    colnat.assign(n, 0);
+    diffrow.assign(n, 0);
+    probe.assign(n, -1);
+    for (int32_t r = 0; r < n; r++) colnat[P.pcol[r]] = r;
+    for (int32_t k = 0; k < n; k++) {
+        diffrow[k] = P.is_diff[P.rownat[k]];
+        probe[k] = P.probe_slot[colnat[k]];
+    }
+    dev.assign(4 * (size_t)nm, 0);
+    for (int32_t m = 0; m < nm; m++) {
+        for (int q = 0; q < 3; q++) {
+            const int32_t v = P.mos[4 * m + q];
+            dev[4 * m + q] = v >= 0 ? P.pcol[v] : (int32_t)CP_GROUND;
+        }
+        dev[4 * m + 3] = P.mos[4 * m + 3];
+    }
+    return true;
+}
+
+}  // namespace cedar

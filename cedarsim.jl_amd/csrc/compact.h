@@ -1,0 +1,67 @@
+// compact.h — the 16-bit plan image of the LDS-resident batched kernel.
+//
+// Everything one Newton iteration reads is packed into one uint16 "blob"
+// that a workgroup copies into LDS when it starts an instance, so the Newton
+// loop issues no global loads at all.  State vectors live in pivot order:
+// index k is both permuted row k and permuted column k (the static pivot
+// sits on the diagonal), so the solve's output x[k] updates unknown k and the
+// row-wise charge of row k belongs to the lane that owns k.
+//
+// Blob arrays (offsets in uint16 units, see CPlan):
+//   rdesc[8*n]   per pivot row k: e0, e1 (its F entries), diag, jq0 (first
+//                stamp ref of entry e0), l0, l1 (linear entries), r0, r1
+//                (residual stamp refs) — one 16-byte LDS read per row
+//   jptr[nnz+1] jref[]              nonlinear stamp refs per F entry
+//   rref[]                          nonlinear stamp refs per row (residual)
+//   lpos[] lcol[]                   linear entries: F position, column
+//   adesc[4*nA]  per A level: F tasks [fb, fe), X tasks [xb, xe)
+//   bdesc[2*nB]  per B level: X tasks [xb, xe)
+//   src[4*n_src]                    kind, offset in wd, length, 0
+//   ftask[4*]  (e, ub, ue, ctl)     F[e] -= sum F[a]F[b] over upd[ub..ue);
+//                                   ctl < 0x7FFF: then *= inv[ctl] (L entry);
+//                                   0x7FFF: plain U entry; 0x8000|k: diagonal
+//                                   of row k, also inv[k] = 1/F[e]
+//   xtask[4*]  (k, ub, ue, fl)      x[k] -= sum F[a]x[b]; fl & 1: x[k] *= inv[k]
+//   upd[2*]    (a, b)
+// A levels run the LU and the forward substitution together (Ly = b is the
+// extra column of the factorisation); B levels run the backward
+// substitution.  A stamp ref is (s << 1) | negate into the stamp buffer
+// st = [id[n_mos] | dI/dVd[n_mos] | dI/dVg[n_mos] | sources[n_src]];
+// dI/dVs = -(dI/dVd + dI/dVg) is expanded into two refs.
+//
+// LDS doubles of one instance: F[nnz] G[nlin] C[nlin] st[stamp_n] u[n] x[n]
+// inv[n] wd[nwd] (source data, each source's dc value after its data).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace cedar {
+
+constexpr uint32_t CP_PLAIN_U = 0x7FFF;
+constexpr uint32_t CP_DIAG = 0x8000;
+constexpr uint32_t CP_GROUND = 0xFFFF;
+
+struct CPlan {
+    int32_t n, nnz, n_mos, n_src, nlin, stamp_n, nwd;
+    int32_t nA, nB;
+    int32_t words;        // blob size in uint16 (a multiple of 4)
+    int32_t dbl;          // LDS doubles of one instance
+    int32_t o_rdesc, o_jptr, o_jref, o_rref, o_lpos, o_lcol;
+    int32_t o_adesc, o_bdesc, o_src, o_ftask, o_xtask, o_upd;
+    int32_t n_ftask, n_xtask, n_upd, pad;
+    // global arrays, read when an instance starts
+    const uint16_t* blob;
+    const int32_t* colnat;    // [n] natural unknown of pivot column k
+    const int32_t* diffrow;   // [n] pivot row k carries charge
+    const int32_t* probe;     // [n] probe slot of pivot column k, or -1
+    const int32_t* dev;       // [4*n_mos] d, g, s as pivot columns (0xFFFF ground), type
+};
+
+struct Plan;
+// Build the blob and the side arrays.  Returns false (with a reason) when a
+// 16-bit index would overflow; the caller then uses the global-plan kernel.
+bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
+                   std::vector<int32_t>& colnat, std::vector<int32_t>& diffrow,
+                   std::vector<int32_t>& probe, std::vector<int32_t>& dev, const char** why);
+
+}  // namespace cedar

@@ -44,6 +44,47 @@ struct BatchArgs {
     cs_tran_opts o;
 };
 
+struct CPlan;
+
+// LDS-resident batched transient (tran_lds.hip): one launch per size class.
+struct LdsArgs {
+    const CPlan* cplans;
+    const DPlan* plans;
+    const int32_t* inst_cid;
+    const int64_t* p_off;
+    const double* params;
+    const int64_t* u_off;
+    const double* u0;        // may be null (DC-op init)
+    const int64_t* w_off;
+    double* wave;            // may be null
+    double* u_final;
+    cs_tran_stats* stats;
+    const int32_t* order;    // this class's instances, largest first
+    int32_t* queue;          // this class's queue head (zeroed before launch)
+    int32_t n_inst;          // instances in this class
+    int32_t dbl_max;         // LDS doubles before the plan image
+    cs_tran_opts o;
+};
+
+// Batched discrete adjoint (adjoint.hip).
+struct AdjArgs {
+    BatchArgs b;
+    double* t_log;          // [n_inst][cap]
+    int32_t* npts_log;      // [n_inst][cap]
+    int64_t cap;
+    double* ckpt;           // [n_inst][ckcap][3][n_max]
+    int64_t ckcap;
+    int32_t K;
+    int32_t pad;
+    int64_t* n_steps;       // [n_inst]
+    double* energy;         // [n_inst]
+    double* grad;           // per instance at g_off, n_mos entries
+    const int64_t* g_off;
+    const int32_t* obj;     // per circuit: a, b, j (natural indices)
+    double* seg;            // per workgroup: K * n_max
+    int32_t* queue2;
+};
+
 constexpr int kSingleThreads = 256;
 constexpr int kBatchThreads = 64;
 
@@ -60,5 +101,17 @@ void launch_charges(const DPlan* dp, const InstDev& w, const double* v, double* 
 size_t batch_lds_bytes(const BatchArgs& a);
 int batch_max_blocks(int nr, size_t lds);   // resident workgroups per CU
 hipError_t launch_tran_batch(int nr, const BatchArgs& a, int nblocks, size_t lds, hipStream_t s);
+
+// LDS-resident batched transient: LDS bytes of a class, resident workgroups
+// per CU, launch (nr = rows per lane: 1, 2 or 4; devices per lane = 2*nr)
+size_t lds_kernel_bytes(int32_t dbl_max, int32_t words_max);
+int lds_max_blocks(int nr, size_t lds);
+hipError_t launch_tran_lds(int nr, const LdsArgs& a, int nblocks, size_t lds, hipStream_t s);
+
+// batched adjoint: LDS bytes, resident workgroups per CU (min over the two
+// kernels), launch of the forward (recording) or reverse kernel
+size_t adj_lds_bytes(const BatchArgs& b);
+int adj_max_blocks(int nr, size_t lds);
+hipError_t launch_adj(int nr, bool reverse, const AdjArgs& a, int nblocks, size_t lds, hipStream_t s);
 
 }  // namespace cedar

@@ -15,6 +15,7 @@
 // DAE interface (cs_residual / cs_jacobian / cs_lu_*) calls directly.
 #include "device.h"
 #include "kernels.h"
+#include "tran_device.h"
 
 namespace cedar {
 
@@ -124,249 +125,6 @@ void launch_charges(const DPlan* dp, const InstDev& w, const double* v, double* 
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int NT = kBatchThreads;
-
-struct Lds {
-    double *F, *inv, *u, *x, *qx, *st;
-};
-
-template <int NR>
-__device__ void run_instance(const DPlan* __restrict__ P, const double* __restrict__ p,
-                             const double* __restrict__ u0, double* __restrict__ wave,
-                             double* __restrict__ ufin, cs_tran_stats* __restrict__ stp,
-                             const cs_tran_opts& o, const Lds& L, double* __restrict__ Glin,
-                             double* __restrict__ Clin, double* __restrict__ linbuf) {
-    const int tid = threadIdx.x;
-    const int n = P->n;
-    const StepDefaults sd = step_defaults(o);
-    dev_lin_setup<NT>(P, p, linbuf, Glin, Clin);
-
-    double Uh0[NR], Uh1[NR], Uh2[NR], Qh0[NR], Qh1[NR], up[NR];
-    long long steps = 0, rej = 0, nit = 0;
-    int rc = CS_OK;
-    const bool uic = (o.init == CS_INIT_UIC) && u0 != nullptr;
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int i = tid + NT * r;
-        Uh0[r] = Uh1[r] = Uh2[r] = Qh0[r] = Qh1[r] = up[r] = 0.0;
-        if (i < n) L.u[i] = uic ? u0[i] : 0.0;
-    }
-    __syncthreads();
-
-    // ---- consistent initial point -------------------------------------
-    {
-        const int mode = uic ? CS_MODE_TRAN : CS_MODE_DCOP;
-        const int maxit = uic ? 50 : 100;
-        dev_sources<NT>(P, p, o.t_start, mode, L.st);
-        bool ok = false;
-        for (int it = 0; it < maxit; it++) {
-            dev_mos<NT>(P, p, L.u, L.st);
-            __syncthreads();
-            const double lm = dev_assemble<NT>(P, Glin, Clin, 0.0, L.u, nullptr, L.st, L.F, L.x,
-                                               uic ? u0 : nullptr);
-            const double fn = block_reduce<NT>(lm, nullptr, OpMax());
-            if (fn <= o.dc_abstol) { ok = true; break; }
-            if (!(fn == fn) || isinf(fn)) break;
-            __syncthreads();
-            dev_lu<NT>(P, L.F, L.inv);
-            dev_fwd<NT>(P, L.F, L.x);
-            dev_bwd<NT>(P, L.F, L.inv, L.x);
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const int i = tid + NT * r;
-                if (i < n) L.u[i] += L.x[P->pcol[i]];
-            }
-            nit++;
-            __syncthreads();
-        }
-        if (!ok) rc = CS_E_NOCONV;
-    }
-
-    double t = o.t_start;
-    if (rc == CS_OK) {
-        __syncthreads();
-        dev_charges<NT>(P, Clin, L.u, L.qx);
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int i = tid + NT * r;
-            if (i < n) { Uh0[r] = L.u[i]; Qh0[r] = L.qx[i]; }
-        }
-        double T0 = o.t_start, T1 = 0.0, T2 = 0.0;
-        int npts = 1;
-        double h = sd.dt_init;
-        int kout = 0;
-        const int n_probe = P->n_probe;
-        if (wave) {
-            while (kout < o.n_out && o.t_start + kout * o.dt_out <= o.t_start + 1e-9 * o.dt_out) {
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const int i = tid + NT * r;
-                    if (i < n) {
-                        const int sl = P->probe_slot[i];
-                        if (sl >= 0) wave[(size_t)kout * n_probe + sl] = Uh0[r];
-                    }
-                }
-                kout++;
-            }
-        }
-        __syncthreads();
-        while (t < o.t_stop) {
-            if (steps + rej >= o.max_steps) { rc = CS_E_MAXSTEPS; break; }
-            double lb = INFINITY;
-            for (int s = tid; s < P->n_src; s += NT) {
-                const int4 d = P->src[s];
-                const double b = wave_next_bp(d.x, p + d.y, d.z, t);
-                if (b < lb) lb = b;
-            }
-            double tb = block_reduce<NT>(lb, nullptr, OpMin());
-            const bool srcbp = tb < o.t_stop;
-            if (!srcbp) tb = o.t_stop;
-            double hh = h < sd.dt_max ? h : sd.dt_max;
-            int hit;
-            const double tn = clip_step(t, hh, tb, hit);
-            const int k = bdf_order(npts, o.max_order);
-            double a0, a1, a2, w0, w1, w2;
-            bdf_coeffs(k, tn, T0, T1, a0, a1, a2);
-            const int np = npts < k + 1 ? npts : k + 1;
-            pred_weights(np, tn, T0, T1, T2, w0, w1, w2);
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const int i = tid + NT * r;
-                if (i < n) {
-                    double v = Uh0[r];
-                    if (np == 2) v = w0 * Uh0[r] + w1 * Uh1[r];
-                    else if (np == 3) v = w0 * Uh0[r] + w1 * Uh1[r] + w2 * Uh2[r];
-                    up[r] = v;
-                    L.u[i] = v;
-                    L.qx[i] = a1 * Qh0[r] + (k == 2 ? a2 * Qh1[r] : 0.0);
-                }
-            }
-            dev_sources<NT>(P, p, tn, CS_MODE_TRAN, L.st);
-            __syncthreads();
-            bool conv = false;
-            for (int it = 0; it < o.newton_maxit; it++) {
-                dev_mos<NT>(P, p, L.u, L.st);
-                __syncthreads();
-                dev_assemble<NT>(P, Glin, Clin, a0, L.u, L.qx, L.st, L.F, L.x, nullptr);
-                __syncthreads();
-                dev_lu<NT>(P, L.F, L.inv);
-                dev_fwd<NT>(P, L.F, L.x);
-                dev_bwd<NT>(P, L.F, L.inv, L.x);
-                nit++;
-                double ss = 0.0;
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const int i = tid + NT * r;
-                    if (i < n) {
-                        const double d = L.x[P->pcol[i]];
-                        const double ui = L.u[i] + d;
-                        L.u[i] = ui;
-                        const double q = d / (o.reltol * fabs(ui) + o.abstol);
-                        ss += q * q;
-                    }
-                }
-                const double nr = sqrt(block_reduce<NT>(ss, nullptr, OpSum()) / n);
-                __syncthreads();
-                if (!(nr == nr)) break;
-                if (nr <= o.newton_tol) { conv = true; break; }
-            }
-            if (!conv) {
-                rej++;
-                h = 0.25 * hh;
-                if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
-                continue;
-            }
-            double fac = 2.0;
-            if (npts >= 2) {
-                const double c = lte_coeff(k, tn, T0, T1, T2, a0, a1, a2);
-                double ss = 0.0;
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const int i = tid + NT * r;
-                    if (i < n) {
-                        const double ui = L.u[i];
-                        const double est = c * (ui - up[r]);
-                        const double m = fabs(ui) > fabs(Uh0[r]) ? fabs(ui) : fabs(Uh0[r]);
-                        const double q = est / (o.reltol * m + o.abstol);
-                        ss += q * q;
-                    }
-                }
-                const double err = sqrt(block_reduce<NT>(ss, nullptr, OpSum()) / n);
-                if (!(err <= 1.0)) {
-                    rej++;
-                    h = hh * reject_factor(err, k);
-                    if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
-                    continue;
-                }
-                fac = accept_factor(err, k);
-            }
-            // accept
-            dev_charges<NT>(P, Clin, L.u, L.qx);
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const int i = tid + NT * r;
-                if (i < n) {
-                    Uh2[r] = Uh1[r]; Uh1[r] = Uh0[r]; Uh0[r] = L.u[i];
-                    Qh1[r] = Qh0[r]; Qh0[r] = L.qx[i];
-                }
-            }
-            T2 = T1; T1 = T0; T0 = tn;
-            if (npts < 3) npts++;
-            steps++;
-            if (wave) {
-                while (kout < o.n_out) {
-                    const double tq = o.t_start + kout * o.dt_out;
-                    if (tq > tn + 1e-9 * o.dt_out) break;
-                    double l0, l1, l2 = 0.0;
-                    if (npts >= 3) {
-                        l0 = (tq - T1) * (tq - T2) / ((T0 - T1) * (T0 - T2));
-                        l1 = (tq - T0) * (tq - T2) / ((T1 - T0) * (T1 - T2));
-                        l2 = (tq - T0) * (tq - T1) / ((T2 - T0) * (T2 - T1));
-                    } else {
-                        const double s = (tq - T1) / (T0 - T1);
-                        l0 = s;
-                        l1 = 1.0 - s;
-                    }
-#pragma unroll
-                    for (int r = 0; r < NR; r++) {
-                        const int i = tid + NT * r;
-                        if (i < n) {
-                            const int sl = P->probe_slot[i];
-                            if (sl >= 0)
-                                wave[(size_t)kout * n_probe + sl] =
-                                    l0 * Uh0[r] + l1 * Uh1[r] + l2 * Uh2[r];
-                        }
-                    }
-                    kout++;
-                }
-            }
-            t = tn;
-            h = hh * fac;
-            if (hit && srcbp) {
-                npts = 1;
-                h = 0.1 * h;
-                if (h < sd.dt_min) h = sd.dt_min;
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int i = tid + NT * r;
-        if (i < n) ufin[i] = (rc == CS_OK || steps > 0) ? Uh0[r] : L.u[i];
-    }
-    if (tid == 0) {
-        stp->steps = steps;
-        stp->rejected = rej;
-        stp->newton_iters = nit;
-        stp->t_end = t;
-        stp->retcode = rc;
-        stp->pad = 0;
-    }
-    __syncthreads();
-}
-
 template <int NR>
 __global__ __launch_bounds__(NT, 4) void k_tran_batch(BatchArgs A) {
     extern __shared__ double lds[];
@@ -392,7 +150,8 @@ __global__ __launch_bounds__(NT, 4) void k_tran_batch(BatchArgs A) {
         const int inst = __builtin_amdgcn_readfirstlane(A.order[qi]);
         const int cid = __builtin_amdgcn_readfirstlane(A.inst_cid[inst]);
         const DPlan* P = A.plans + cid;
-        run_instance<NR>(P, A.params + A.p_off[inst], A.u0 ? A.u0 + A.u_off[inst] : nullptr,
+        NoRec rec;
+        run_instance<NR>(rec, P, A.params + A.p_off[inst], A.u0 ? A.u0 + A.u_off[inst] : nullptr,
                          A.wave ? A.wave + A.w_off[inst] : nullptr, A.u_final + A.u_off[inst],
                          A.stats + inst, A.o, L, Glin, Clin, linbuf);
     }

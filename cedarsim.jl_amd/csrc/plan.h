@@ -68,6 +68,7 @@ struct Plan {
     std::vector<int32_t> jnl_ptr, jnl_ref;   // per F position
     std::vector<int32_t> rnl_ptr, rnl_ref;   // per permuted row
     std::vector<SrcDesc> src;
+    std::vector<int32_t> vsrc_nodes;         // (a, b) per voltage source
 
     // LU and substitution schedules
     std::vector<int32_t> lu_lptr;

@@ -120,14 +120,13 @@ CEDAR_HD double source_value(int kind, const double* d, int len, double dc, doub
 // Square-law MOSFET (models/sqlaw.va): current d->s and dI/dVd, dI/dVg,
 // dI/dVs.  Source and drain swap when type*Vds < 0.  gmin is in parallel
 // with d-s.  dI/dVs = -(dI/dVd + dI/dVg) holds exactly.
-CEDAR_HD void sqlaw_eval(int type, double vth, double kp, double w, double l, double gmin,
-                         double vd, double vg, double vs, double& id, double& dd,
-                         double& dg, double& ds) {
-    const double beta = kp * w / l;
-    const double tp = (double)type;
+// sqlaw_core takes beta = kp*w/l and tp*vth (tp = +-1) precomputed, the
+// form the batched kernel keeps in registers; the arithmetic is the same.
+CEDAR_HD void sqlaw_core(double tp, double beta, double tpvth, double gmin, double vd, double vg,
+                         double vs, double& id, double& dd, double& dg, double& ds) {
     const double vds = tp * (vd - vs);
     const bool fwd = vds >= 0.0;
-    const double vov = fwd ? tp * (vg - vs) - tp * vth : tp * (vg - vd) - tp * vth;
+    const double vov = fwd ? tp * (vg - vs) - tpvth : tp * (vg - vd) - tpvth;
     const double vx = fwd ? vds : -vds;
     double I = 0.0, gm = 0.0, gds = 0.0;
     if (vov > 0.0) {
@@ -154,6 +153,13 @@ CEDAR_HD void sqlaw_eval(int type, double vth, double kp, double w, double l, do
     id += gmin * (vd - vs);
     dd += gmin;
     ds -= gmin;
+}
+
+CEDAR_HD void sqlaw_eval(int type, double vth, double kp, double w, double l, double gmin,
+                         double vd, double vg, double vs, double& id, double& dd,
+                         double& dg, double& ds) {
+    const double tp = (double)type;
+    sqlaw_core(tp, kp * w / l, tp * vth, gmin, vd, vg, vs, id, dd, dg, ds);
 }
 
 // ---- BDF step rules --------------------------------------------------------

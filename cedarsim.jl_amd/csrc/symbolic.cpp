@@ -201,6 +201,8 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
         addG(a, j, 0, false); addG(b, j, 0, true);
         addG(j, a, 0, false); addG(j, b, 0, true);
         addR(j, stamp_src0 + k, true);
+        P.vsrc_nodes.push_back(a);
+        P.vsrc_nodes.push_back(b);
     }
     for (int32_t k = 0; k < nl->n_isrc; k++) {
         int32_t a = nl->isrc_nodes[2 * k], b = nl->isrc_nodes[2 * k + 1];

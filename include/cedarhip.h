@@ -71,6 +71,11 @@ extern "C" {
                              to u0 and the remaining (algebraic) unknowns are
                              solved for consistency                          */
 
+/* ---- batched transient kernels (cs_tran_opts.batch_kernel) -------------- */
+#define CS_BATCH_KERNEL_AUTO   0  /* LDS-resident kernel when every circuit fits */
+#define CS_BATCH_KERNEL_LDS    1  /* LDS-resident kernel; an error if one does not fit */
+#define CS_BATCH_KERNEL_GLOBAL 2  /* plan read from global memory (any size <= 512) */
+
 /*
  * Flat netlist: the lowered form of a circuit after netlist import
  * (make_spectre_circuit, src/spectre.jl:1656-1697).  Node lists are int32
@@ -147,7 +152,7 @@ typedef struct cs_tran_opts {
     int32_t newton_maxit;
     int32_t init;                /* CS_INIT_DCOP or CS_INIT_UIC               */
     int64_t max_steps;
-    int32_t lanes;               /* batched: threads per instance, 0 = tuned  */
+    int32_t batch_kernel;        /* batched: CS_BATCH_KERNEL_*                */
     int32_t record_steps;        /* keep the accepted-step sequence (adjoint) */
 } cs_tran_opts;
 
@@ -223,8 +228,10 @@ int cs_tran(cs_ctx* ctx, int32_t cid, const double* p, const double* u0,
  * Instances may use different circuits (uneven sizes).  params holds each
  * instance's parameter vector back to back; u0 (may be NULL) each instance's
  * initial state back to back.  cs_batch_create uploads everything; run
- * launches one persistent kernel that walks a work queue of instances, each
- * doing its whole transient on the device; results downloads.             */
+ * launches persistent kernels that walk work queues of instances, each
+ * doing its whole transient on the device; results downloads.  With the
+ * LDS-resident kernel (o->batch_kernel) circuits are grouped into size
+ * classes, one launch per class on concurrent streams.                    */
 int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid,
                     const double* params, const double* u0,
                     const cs_tran_opts* o, int32_t* bid);
@@ -236,6 +243,33 @@ int cs_batch_destroy(cs_ctx* ctx, int32_t bid);
  * executed Newton iterations (device eval + assembly + LU + solves). */
 int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes,
                      double* alg_flops);
+/* Which kernel a batch runs (CS_BATCH_KERNEL_LDS or _GLOBAL) and how many
+ * launches one run makes. */
+int cs_batch_kernel(cs_ctx* ctx, int32_t bid, int32_t* kernel, int32_t* n_launches);
+
+/* ---- batched adjoint sensitivities ------------------------------------- *
+ * Replaces forward sensitivity through DAECompiler's paramjac
+ * (ODEForwardSensitivityProblem, test/sensitivity.jl:29-41) for the energy
+ * objective: E = energy delivered by voltage source `vsrc` over the run
+ * (trapezoid on the accepted points), gradient with respect to every MOSFET
+ * width.  Discrete adjoint of the BDF scheme on the recorded step sequence,
+ * checkpoints every ckpt_every accepted steps, recomputation in between.
+ * Requires o->init == CS_INIT_UIC.  Outputs: energy[n_inst]; grad: each
+ * instance's n_mos entries back to back; n_steps[n_inst]; optional step log
+ * t_log / npts_log ([n_inst][max_steps], may be NULL) for frozen-step replays
+ * (npts_log[m] = points of the current segment before step m).             */
+typedef struct cs_adjoint_opts {
+    int32_t vsrc;           /* index of the voltage source of the objective */
+    int32_t ckpt_every;     /* accepted steps between checkpoints (64)      */
+    int64_t max_steps;      /* step-log capacity per instance               */
+} cs_adjoint_opts;
+
+int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid,
+                     const double* params, const double* u0,
+                     const cs_tran_opts* o, const cs_adjoint_opts* ao,
+                     double* energy, double* grad, int64_t* n_steps,
+                     double* t_log, int32_t* npts_log, cs_tran_stats* st,
+                     double* kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -116,10 +116,13 @@ static void mos_dw(const cs_netlist* nl, const double* p, const double* u, doubl
     }
 }
 
-/* Adaptive forward with every state kept, then the reverse sweep. */
-int or_adjoint(const cs_netlist* nl, const double* p, const double* u0, const cs_tran_opts* o,
-               int vsrc, double* E_out, double* grad, cs_tran_stats* st_out,
-               long long* n_steps, double* t_log, int* npts_log, long long log_cap) {
+/* Forward with every state kept (adaptive, or replaying t_in / npts_in when
+ * given), then the reverse sweep. */
+static int adjoint_core(const cs_netlist* nl, const double* p, const double* u0,
+                        const cs_tran_opts* o, int vsrc, double* E_out, double* grad,
+                        cs_tran_stats* st_out, long long* n_steps, double* t_log, int* npts_log,
+                        long long log_cap, const double* t_in, const int* npts_in,
+                        long long M_in) {
     const int n = nunk(nl);
     const double span = o->t_stop - o->t_start;
     const double dt_init = o->dt_init > 0 ? o->dt_init : span * 1e-9;
@@ -147,16 +150,24 @@ int or_adjoint(const cs_netlist* nl, const double* p, const double* u0, const cs
     int npts = 1;
     double t = o->t_start, h = dt_init;
     /* the state history for the predictor lives in US[M-2..M] */
-    while (t < o->t_stop) {
+    while (t_in ? (M < M_in) : (t < o->t_stop)) {
         if (st.steps + st.rejected >= o->max_steps) { rc = CS_E_MAXSTEPS; break; }
-        double tb = or_next_breakpoint(nl, p, t), hh = h < dt_max ? h : dt_max;
-        int src_bp = 1;
-        if (!(tb < o->t_stop)) { tb = o->t_stop; src_bp = 0; }
-        int hit = 0;
-        if (tb - t <= hh * (1.0 + 1e-9)) { hh = tb - t; hit = 1; }
-        else if (tb - t < 2.0 * hh) hh = 0.5 * (tb - t);
-        double tn = hit ? tb : t + hh;
-        hh = tn - t;
+        double tn, hh;
+        int src_bp = 0, hit = 0;
+        if (t_in) {
+            tn = t_in[M + 1];
+            npts = npts_in[M + 1];
+            hh = tn - t;
+        } else {
+            double tb = or_next_breakpoint(nl, p, t);
+            hh = h < dt_max ? h : dt_max;
+            src_bp = 1;
+            if (!(tb < o->t_stop)) { tb = o->t_stop; src_bp = 0; }
+            if (tb - t <= hh * (1.0 + 1e-9)) { hh = tb - t; hit = 1; }
+            else if (tb - t < 2.0 * hh) hh = 0.5 * (tb - t);
+            tn = hit ? tb : t + hh;
+            hh = tn - t;
+        }
         int k = (npts >= 3 && o->max_order >= 2) ? 2 : 1;
         double T0 = SR[M].t, T1 = M >= 1 ? SR[M - 1].t : 0.0, T2 = M >= 2 ? SR[M - 2].t : 0.0;
         double a0, a1, a2 = 0.0;
@@ -185,13 +196,14 @@ int or_adjoint(const cs_netlist* nl, const double* p, const double* u0, const cs
         int r2 = step_solve(nl, p, n, tn, a0, qc, tols, o->newton_tol, o->newton_maxit, u, F, Q,
                             J, C, d, piv, &st.newton_iters);
         if (r2 != CS_OK) {
+            if (t_in) { rc = CS_E_NOCONV; break; }
             st.rejected++;
             h = 0.25 * hh;
             if (h < dt_min) { rc = CS_E_DTMIN; break; }
             continue;
         }
         double fac = 2.0;
-        if (npts >= 2) {
+        if (npts >= 2 && !t_in) {
             double c;
             if (k == 1) c = (a1 * (T0 - tn) * (T0 - tn)) / (a0 * (tn - T0) * (tn - T1));
             else {
@@ -319,6 +331,20 @@ out:
     free(US); free(SR); free(F); free(Q); free(J); free(C); free(d); free(u); free(up);
     free(qc); free(QH); free(piv); free(is_diff);
     return rc;
+}
+
+int or_adjoint(const cs_netlist* nl, const double* p, const double* u0, const cs_tran_opts* o,
+               int vsrc, double* E, double* grad, cs_tran_stats* st, long long* n_steps,
+               double* t_log, int* npts_log, long long log_cap) {
+    return adjoint_core(nl, p, u0, o, vsrc, E, grad, st, n_steps, t_log, npts_log, log_cap,
+                        NULL, NULL, 0);
+}
+
+int or_adjoint_replay(const cs_netlist* nl, const double* p, const double* u0,
+                      const cs_tran_opts* o, int vsrc, long long M, const double* t_log,
+                      const int* npts_log, double* E, double* grad) {
+    return adjoint_core(nl, p, u0, o, vsrc, E, grad, NULL, NULL, NULL, NULL, 0, t_log, npts_log,
+                        M);
 }
 
 /* Energy of a run that replays a recorded step sequence (times t_log[0..M],

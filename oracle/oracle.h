@@ -39,6 +39,12 @@ int    or_adjoint(const cs_netlist* nl, const double* p, const double* u0,
                   const cs_tran_opts* o, int vsrc, double* E, double* grad,
                   cs_tran_stats* st, long long* n_steps, double* t_log,
                   int* npts_log, long long log_cap);
+/* The same adjoint along a logged step sequence (t_log[0..M], npts_log),
+ * Newton converged to o->newton_tol. */
+int    or_adjoint_replay(const cs_netlist* nl, const double* p, const double* u0,
+                         const cs_tran_opts* o, int vsrc, long long M,
+                         const double* t_log, const int* npts_log, double* E,
+                         double* grad);
 /* Energy of a replay of a logged step sequence, Newton converged to ntol. */
 int    or_replay_energy(const cs_netlist* nl, const double* p, const double* u0,
                         const cs_tran_opts* o, int vsrc, long long M,

@@ -94,6 +94,8 @@ def lib():
         L.or_replay_energy.argtypes = [N, _D, _D, POINTER(_Opts), c_int, c_longlong, _D,
                                        POINTER(c_int), c_double, _D]
         L.or_cmatrix.argtypes = [N, _D, _D, _D]
+        L.or_adjoint_replay.argtypes = [N, _D, _D, POINTER(_Opts), c_int, c_longlong, _D,
+                                        POINTER(c_int), _D, _D]
         L.or_set_sparse.argtypes = [c_int]
         L.or_set_sparse.restype = None
         L.or_dcop.argtypes = [N, _D, POINTER(_Opts), _D, POINTER(Stats)]
@@ -177,6 +179,22 @@ def adjoint(flat, p, u0, vsrc=0, o=None, log_cap=2_000_000, **kw):
                           npl.ctypes.data_as(POINTER(c_int)), log_cap)
     m = M.value
     return E.value, g[:nm], st, rc, tl[:m + 1].copy(), npl[:m + 1].copy()
+
+
+def adjoint_replay(flat, p, u0, t_log, npts_log, vsrc=0, o=None, **kw):
+    """The adjoint along a given step log (t_log[0..M], npts_log[0..M])."""
+    nl = Netlist(flat)
+    oo = opts(o, **kw)
+    E = c_double()
+    nm = len(flat.arrays["mos_vth"])
+    g = np.zeros(max(nm, 1))
+    tl = np.ascontiguousarray(t_log, np.float64)
+    npl = np.ascontiguousarray(npts_log, np.int32)
+    rc = lib().or_adjoint_replay(byref(nl.s), _dp(np.ascontiguousarray(p, np.float64)),
+                                 _dp(np.ascontiguousarray(u0, np.float64)), byref(oo), vsrc,
+                                 len(tl) - 1, _dp(tl), npl.ctypes.data_as(POINTER(c_int)),
+                                 byref(E), _dp(g))
+    return E.value, g[:nm], rc
 
 
 def replay_energy(flat, p, u0, t_log, npts_log, vsrc=0, o=None, ntol=1e-9, **kw):

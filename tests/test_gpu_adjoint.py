@@ -1,0 +1,91 @@
+"""GPU adjoint (cs_adjoint_batch) against the oracle and finite differences.
+
+The GPU records its accepted-step sequence; the oracle replays exactly that
+sequence (same discrete scheme) for (a) its own adjoint and (b) central finite
+differences in each width (h = 1e-6 relative).  Bars: energy to 1e-9
+relative, gradients to 1e-6 against the oracle adjoint and 1e-5 against the
+finite differences.
+"""
+import numpy as np
+import pytest
+
+import cedarhip as ch
+from cedarhip.workloads import ring_initial_state, ring_oscillator
+from oracle import oracle as O
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="the batched adjoint kernels are switched off: a GPU run of this test "
+                            "stalled (killed after 180 s without output) while the same kernel "
+                            "code finishes in seconds under CPU emulation; no cause found yet"),
+]
+
+T_STOP = 0.3e-9
+
+
+def _say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _batch(sizes, seed):
+    rng = np.random.default_rng(seed)
+    cache, flats, params, u0s = {}, [], [], []
+    for N in sizes:
+        if N not in cache:
+            cache[N] = ch.lower(ring_oscillator(N))
+        f = cache[N]
+        p = f.params()
+        lay = f.layout()
+        nm = len(f.arrays["mos_w"])
+        p[lay["mos_w"]:lay["mos_w"] + nm] = rng.uniform(0.5e-6, 4e-6, nm)
+        flats.append(f)
+        params.append(p)
+        u0s.append(ring_initial_state(f, rng))
+    return flats, params, u0s
+
+
+def _opts():
+    return ch.TranOptions(t_start=0.0, t_stop=T_STOP, reltol=1e-6, abstol=1e-9,
+                          newton_tol=1e-9, newton_maxit=30, init=1)
+
+
+def test_adjoint_matches_oracle_and_fd(engine):
+    _say("adjoint rings 5/7/11, 0.3 ns")
+    flats, params, u0s = _batch([5, 7, 11, 5], seed=4)
+    o = _opts()
+    res = ch.adjoint_batch(flats, params, u0s, o, vsrc=0, ckpt_every=64, keep_logs=True,
+                           engine=engine)
+    assert all(r == "Success" for r in res.retcodes), res.retcodes
+    oo = dict(t_start=0.0, t_stop=T_STOP, reltol=1e-6, abstol=1e-9, newton_tol=1e-9,
+              newton_maxit=30, init=1)
+    for i, f in enumerate(flats):
+        tl, npl = res.t_logs[i], res.npts_logs[i]
+        E_or, g_or, rc = O.adjoint_replay(f, params[i], u0s[i], tl, npl, 0, **oo)
+        assert rc == 0
+        assert res.energy[i] == pytest.approx(E_or, rel=1e-9)
+        np.testing.assert_allclose(res.grads[i], g_or, rtol=1e-6, atol=1e-6 * np.max(np.abs(g_or)))
+        lay = f.layout()
+        for k in (0, 1, len(g_or) - 1):
+            p = params[i]
+            hW = 1e-6 * p[lay["mos_w"] + k]
+            pp, pm = p.copy(), p.copy()
+            pp[lay["mos_w"] + k] += hW
+            pm[lay["mos_w"] + k] -= hW
+            Ep, _ = O.replay_energy(f, pp, u0s[i], tl, npl, 0, **oo)
+            Em, _ = O.replay_energy(f, pm, u0s[i], tl, npl, 0, **oo)
+            fd = (Ep - Em) / (2 * hW)
+            assert res.grads[i][k] == pytest.approx(fd, rel=1e-5), (i, k)
+
+
+def test_adjoint_checkpoint_interval_and_determinism(engine):
+    _say("adjoint checkpoint interval 64 vs 7, repeat")
+    flats, params, u0s = _batch([7, 9], seed=8)
+    o = _opts()
+    r64 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, engine=engine)
+    r7 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=7, engine=engine)
+    r64b = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, engine=engine)
+    for i in range(len(flats)):
+        assert np.array_equal(r64.grads[i], r64b.grads[i]), "adjoint is not bit-reproducible"
+        assert r64.energy[i] == r64b.energy[i]
+        np.testing.assert_allclose(r7.grads[i], r64.grads[i], rtol=1e-10,
+                                   atol=1e-12 * np.max(np.abs(r64.grads[i])))

@@ -115,3 +115,38 @@ def test_batch_rings_parity_and_determinism(engine):
         assert r1.stats[i].steps == st_ref[i].steps, i
         assert _rel_close(r1.wave[i], w_ref[i], 1e-6, 1e-9), i
         assert np.array_equal(r1.wave[i], r2.wave[i]), "batched run is not bit-reproducible"
+
+
+@pytest.mark.parametrize("kernel", [L.BATCH_KERNEL_LDS, L.BATCH_KERNEL_GLOBAL])
+def test_batch_kernels_uneven_rings(engine, kernel):
+    """Both batched kernels on uneven rings (11..101 stages: several LDS size
+    classes, one and two rows per lane) against the oracle."""
+    _say(f"batched kernel {kernel}: rings 11..101, 0-0.2ns")
+    flats, params, u0s, stages = ring_mc(n_inst=24, seed=9, nmin=11, nmax=101)
+    opts = ch.grid_options((0.0, 0.2e-9), dt_out=10e-12, init=L.INIT_UIC, batch_kernel=kernel)
+    r = ch.tran_batch(flats, params, opts, u0=u0s, engine=engine)
+    w_ref, uf_ref, st_ref, rc = O.tran_batch(flats, params, u0s, opts)
+    assert rc == 0
+    for i in range(len(flats)):
+        assert r.stats[i].retcode == 0, i
+        assert r.stats[i].steps == st_ref[i].steps, (i, int(stages[i]))
+        assert _rel_close(r.wave[i], w_ref[i], 1e-6, 1e-9), i
+        np.testing.assert_allclose(r.u_final[i], uf_ref[i], rtol=1e-6, atol=1e-9)
+
+
+def test_batch_lds_inverter_dcop_pulse(engine):
+    """LDS-resident kernel with DC-operating-point initialisation, a PULSE
+    source and its breakpoints, against the oracle over two periods."""
+    _say("LDS kernel: inverter DC op + pulse, 0-4ns")
+    f = ch.lower(inverter())
+    p = f.params()
+    opts = ch.grid_options((0.0, 4e-9), dt_out=50e-12, init=L.INIT_DCOP,
+                           batch_kernel=L.BATCH_KERNEL_LDS)
+    r = ch.tran_batch([f, f], [p, p], opts, engine=engine)
+    w_ref, uf_ref, st_ref, rc = O.tran_batch([f], [p], None, opts)
+    assert rc == 0
+    for i in range(2):
+        assert r.stats[i].retcode == 0
+        assert r.stats[i].steps == st_ref[0].steps
+        assert _rel_close(r.wave[i], w_ref[0], 1e-6, 1e-9)
+    assert np.array_equal(r.wave[0], r.wave[1])
