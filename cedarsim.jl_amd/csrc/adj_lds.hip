@@ -1,0 +1,396 @@
+// adj_lds.hip — batched discrete adjoint on the LDS-resident kernel.
+//
+// Objective per instance: energy delivered by one voltage source,
+//     E = sum_m w_m P(u_m),  P(u) = -(V(a) - V(b)) * I_src,
+// trapezoid weights on the accepted time points; parameters: every MOSFET
+// width.  Two persistent kernels per size class, one wavefront per instance,
+// the circuit's plan image in LDS (lds_core.h):
+//
+//   k_adj_lds_forward  the transient of run_lds with a recording policy:
+//                      every accepted time and segment count goes to a step
+//                      log, E is integrated, and every K accepted steps the
+//                      three states the predictor needs are checkpointed;
+//   k_adj_lds_reverse  walks the segments [sK, (s+1)K) from the last one
+//                      back: restores checkpoint s, recomputes the segment's
+//                      states from the log (same predictor and Newton, so
+//                      the same bits), then for m descending solves
+//                        J_m^T l_m = -( w_m dP/du(u_m) + a_{m+1,1} C^T l_{m+1}
+//                                       + a_{m+2,2} C^T l_{m+2} )
+//                      on J_m's LU with the transposed level schedules, and
+//                      accumulates dE/dW_k += (l_m[d] - l_m[s]) * I_ch,k / W_k.
+// Step 0 is the consistent-initialisation system (charge rows pinned), so
+// the adjoint needs UIC.  C^T l is carried between steps in the registers of
+// the lanes that own the columns and each device's gradient is accumulated
+// by one lane in a fixed order, so results are bit-reproducible.
+#include "lds_core.h"
+
+namespace cedar {
+
+namespace {
+
+using namespace lds;
+
+struct AdjRec {
+    double* t_log;
+    int32_t* npts_log;
+    long long cap;
+    double* ckpt;           // this instance's checkpoints
+    long long ckcap;
+    int K, ck_n;
+    uint32_t a, b, j;       // objective source: terminals and branch current (pivot columns)
+    double E, Pprev;
+    long long M;
+
+    template <int NR>
+    __device__ __forceinline__ bool on_accept(const CPlan& c, const View& v, double tn, double tprev,
+                                              int npts, long long m, const double (&Uh0)[NR],
+                                              const double (&Uh1)[NR], const double (&Uh2)[NR]) {
+        if (m >= cap) return false;
+        if (lane_id() == 0) {
+            t_log[m] = tn;
+            npts_log[m] = npts;
+        }
+        const double Pn = -((a != CP_GROUND ? v.u[a] : 0.0) - (b != CP_GROUND ? v.u[b] : 0.0)) * v.u[j];
+        if (m > 0) E += 0.5 * (tn - tprev) * (Pn + Pprev);
+        Pprev = Pn;
+        M = m;
+        if (m % K == 0) {
+            const long long s = m / K;
+            if (s >= ckcap) return false;
+            double* cp = ckpt + (size_t)s * 3 * ck_n;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int k = lane_id() + W * r;
+                if (k < c.n) {
+                    cp[k] = Uh0[r];
+                    cp[ck_n + k] = Uh1[r];
+                    cp[2 * ck_n + k] = Uh2[r];
+                }
+            }
+        }
+        return true;
+    }
+};
+
+__device__ __forceinline__ void step_coeffs(const double* tl, const int32_t* npl, long long q,
+                                            int max_order, double& a0, double& a1, double& a2) {
+    const int k = bdf_order(npl[q], max_order);
+    bdf_coeffs(k, tl[q], tl[q - 1], q >= 2 ? tl[q - 2] : 0.0, a0, a1, a2);
+}
+
+template <int NR>
+__global__ __launch_bounds__(W) void k_adj_lds_forward(AdjLdsArgs A) {
+    extern __shared__ double lds_mem[];
+    const LdsArgs& F = A.f;
+    uint16_t* blob = reinterpret_cast<uint16_t*>(lds_mem + F.dbl_max);
+    int cur = -1;
+    for (;;) {
+        int qi = 0;
+        if (threadIdx.x == 0) qi = atomicAdd(F.queue, 1);
+        qi = __builtin_amdgcn_readfirstlane(__shfl(qi, 0, W));
+        if (qi >= F.n_inst) break;
+        const int inst = __builtin_amdgcn_readfirstlane(F.order[qi]);
+        const int cid = __builtin_amdgcn_readfirstlane(F.inst_cid[inst]);
+        const CPlan c = load_header(F.cplans + cid);
+        if (cid != cur) {
+            copy_image(c, blob);
+            cur = cid;
+            __syncthreads();
+        }
+        const View v = make_view(lds_mem, blob, c);
+        AdjRec rec;
+        rec.t_log = A.t_log + (size_t)inst * A.cap;
+        rec.npts_log = A.npts_log + (size_t)inst * A.cap;
+        rec.cap = A.cap;
+        rec.ckpt = A.ckpt + (size_t)inst * A.ckcap * 3 * A.ck_n;
+        rec.ckcap = A.ckcap;
+        rec.K = A.K;
+        rec.ck_n = A.ck_n;
+        rec.a = (uint32_t)A.obj[3 * cid];
+        rec.b = (uint32_t)A.obj[3 * cid + 1];
+        rec.j = (uint32_t)A.obj[3 * cid + 2];
+        rec.E = 0.0;
+        rec.Pprev = 0.0;
+        rec.M = 0;
+        run_lds<NR>(rec, c, F.plans + cid, F.params + F.p_off[inst], F.u0 + F.u_off[inst], nullptr,
+                    F.u_final + F.u_off[inst], F.stats + inst, F.o, v);
+        if (threadIdx.x == 0) {
+            A.energy[inst] = rec.E;
+            A.n_steps[inst] = rec.M;
+        }
+    }
+}
+
+template <int NR>
+__device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __restrict__ P,
+                            const double* __restrict__ p, const double* __restrict__ u0, int inst,
+                            int cid, const View& v, double* __restrict__ seg) {
+    constexpr int NM = 2 * NR;
+    const int tid = lane_id();
+    const int n = c.n;
+    const cs_tran_opts& o = A.f.o;
+    const long long M = A.n_steps[inst];
+    const int K = A.K;
+    const int ck_n = A.ck_n;
+    const double* tl = A.t_log + (size_t)inst * A.cap;
+    const int32_t* npl = A.npts_log + (size_t)inst * A.cap;
+    const double* ck = A.ckpt + (size_t)inst * A.ckcap * 3 * ck_n;
+    double* grad = A.grad + A.g_off[inst];
+    const uint32_t oa = (uint32_t)A.obj[3 * cid], ob = (uint32_t)A.obj[3 * cid + 1],
+                   oj = (uint32_t)A.obj[3 * cid + 2];
+
+    DevRegs<NM> dr;
+    setup_instance<NM>(c, P, p, v, dr);
+    const double gmin = p[P->p_gmin];
+    // per device slot: width, drain / source pivot rows (and whether those
+    // rows are pinned at step 0), gradient accumulator
+    double wdt[NM], gacc[NM];
+    uint32_t drow[NM], srow[NM], pin0[NM];
+#pragma unroll
+    for (int jd = 0; jd < NM; jd++) {
+        const int m = tid + W * jd;
+        wdt[jd] = 1.0;
+        gacc[jd] = 0.0;
+        drow[jd] = srow[jd] = CP_GROUND;
+        pin0[jd] = 0;
+        if (m < c.n_mos) {
+            wdt[jd] = p[P->p_mos_w + m];
+            drow[jd] = (uint32_t)c.devrow[2 * m];
+            srow[jd] = (uint32_t)c.devrow[2 * m + 1];
+            if (drow[jd] != CP_GROUND && c.diffrow[drow[jd]]) pin0[jd] |= 1;
+            if (srow[jd] != CP_GROUND && c.diffrow[srow[jd]]) pin0[jd] |= 2;
+        }
+    }
+    double pinv[NR], ctl1[NR], ctl2[NR];
+    uint32_t flag[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int k = tid + W * r;
+        pinv[r] = ctl1[r] = ctl2[r] = 0.0;
+        flag[r] = 0;
+        if (k < n) {
+            pinv[r] = u0[c.colnat[k]];
+            flag[r] = c.diffrow[k] ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    long long nit = 0;
+
+    for (long long s = M / K; s >= 0; s--) {
+        const long long m0 = s * K;
+        const long long m1 = (m0 + K < M + 1) ? m0 + K : M + 1;
+        double U0[NR], U1[NR], U2[NR], Q0[NR], Q1[NR], qx[NR];
+        const double* cp = ck + (size_t)s * 3 * ck_n;
+        __syncthreads();   // the previous segment's last step still reads u
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int k = tid + W * r;
+            U0[r] = U1[r] = U2[r] = Q0[r] = Q1[r] = qx[r] = 0.0;
+            if (k < n) {
+                U0[r] = cp[k];
+                U1[r] = cp[ck_n + k];
+                U2[r] = cp[2 * ck_n + k];
+                seg[k] = U0[r];
+                v.u[k] = U1[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int k = tid + W * r;
+            if (k < n) Q1[r] = row_charge(c, v, k);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int k = tid + W * r;
+            if (k < n) v.u[k] = U0[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int k = tid + W * r;
+            if (k < n) Q0[r] = row_charge(c, v, k);
+        }
+        // ---- recompute the segment's states from the log -------------------
+        for (long long m = m0 + 1; m < m1; m++) {
+            const int npts = npl[m];
+            const int k = bdf_order(npts, o.max_order);
+            const double tn = tl[m], T0 = tl[m - 1];
+            const double T1 = m >= 2 ? tl[m - 2] : 0.0, T2 = m >= 3 ? tl[m - 3] : 0.0;
+            double a0, a1, a2, w0, w1, w2;
+            bdf_coeffs(k, tn, T0, T1, a0, a1, a2);
+            const int np = npts < k + 1 ? npts : k + 1;
+            pred_weights(np, tn, T0, T1, T2, w0, w1, w2);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int kk = tid + W * r;
+                if (kk < n) {
+                    double val = U0[r];
+                    if (np == 2) val = w0 * U0[r] + w1 * U1[r];
+                    else if (np == 3) val = w0 * U0[r] + w1 * U1[r] + w2 * U2[r];
+                    v.u[kk] = val;
+                    qx[r] = a1 * Q0[r] + (k == 2 ? a2 * Q1[r] : 0.0);
+                }
+            }
+            eval_sources(c, v, tn, CS_MODE_TRAN);
+            __syncthreads();
+            newton<NR, NM>(c, v, dr, gmin, a0, qx, pinv, flag, o.newton_maxit, o.newton_tol,
+                           o.reltol, o.abstol, nit);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int kk = tid + W * r;
+                if (kk < n) {
+                    U2[r] = U1[r]; U1[r] = U0[r]; U0[r] = v.u[kk];
+                    Q1[r] = Q0[r]; Q0[r] = row_charge(c, v, kk);
+                    seg[(size_t)(m - m0) * ck_n + kk] = U0[r];
+                }
+            }
+        }
+        // ---- backward over the segment ------------------------------------
+        for (long long m = m1 - 1; m >= m0; m--) {
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int k = tid + W * r;
+                if (k < n) v.u[k] = seg[(size_t)(m - m0) * ck_n + k];
+            }
+            const double tm = tl[m];
+            double w = 0.0;
+            if (m > 0) w += 0.5 * (tm - tl[m - 1]);
+            if (m < M) w += 0.5 * (tl[m + 1] - tm);
+            double c1 = 0.0, c2 = 0.0, b0, b1, b2;
+            if (m + 1 <= M) { step_coeffs(tl, npl, m + 1, o.max_order, b0, b1, b2); c1 = b1; }
+            if (m + 2 <= M) { step_coeffs(tl, npl, m + 2, o.max_order, b0, b1, b2); c2 = b2; }
+            double am0 = 0.0;
+            if (m > 0) step_coeffs(tl, npl, m, o.max_order, am0, b1, b2);
+            eval_sources(c, v, tm, CS_MODE_TRAN);
+            __syncthreads();
+            eval_mos<NM>(c, v, dr, gmin);
+            __syncthreads();
+            double qz[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) qz[r] = 0.0;
+            assemble<NR>(c, v, am0, qz, m == 0, pinv, flag);
+            __syncthreads();
+            factor<false>(c, v);
+            const double ua = oa != CP_GROUND ? v.u[oa] : 0.0, ub = ob != CP_GROUND ? v.u[ob] : 0.0;
+            const double uj = v.u[oj];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int k = tid + W * r;
+                if (k < n) {
+                    double dP = 0.0;
+                    if ((uint32_t)k == oa) dP -= uj;
+                    if ((uint32_t)k == ob) dP += uj;
+                    if ((uint32_t)k == oj) dP -= (ua - ub);
+                    v.x[k] = -w * dP - c1 * ctl1[r] - c2 * ctl2[r];
+                }
+            }
+            __syncthreads();
+            solve_transposed(c, v);
+            // C^T lambda for this lane's columns
+            double ctl[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int k = tid + W * r;
+                ctl[r] = k < n ? col_charge_t(c, v, v.x, k) : 0.0;
+            }
+            // dE/dW: lambda of the drain and source rows times I_ch / W
+#pragma unroll
+            for (int jd = 0; jd < NM; jd++) {
+                const int md = tid + W * jd;
+                if (md < c.n_mos) {
+                    const uint32_t pz = m == 0 ? pin0[jd] : 0u;
+                    const double ld = (drow[jd] != CP_GROUND && !(pz & 1)) ? v.x[drow[jd]] : 0.0;
+                    const double ls = (srow[jd] != CP_GROUND && !(pz & 2)) ? v.x[srow[jd]] : 0.0;
+                    const uint32_t dcol = dr.dg[jd] & 0xFFFF, scol = dr.sx[jd] & 0xFFFF;
+                    const double vd = dcol != CP_GROUND ? v.u[dcol] : 0.0;
+                    const double vs = scol != CP_GROUND ? v.u[scol] : 0.0;
+                    const double ich = v.st[md] - gmin * (vd - vs);
+                    gacc[jd] += (ld - ls) * ich / wdt[jd];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                ctl2[r] = ctl1[r];
+                ctl1[r] = ctl[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int jd = 0; jd < NM; jd++) {
+        const int md = tid + W * jd;
+        if (md < c.n_mos) grad[md] = gacc[jd];
+    }
+    __syncthreads();
+}
+
+template <int NR>
+__global__ __launch_bounds__(W) void k_adj_lds_reverse(AdjLdsArgs A) {
+    extern __shared__ double lds_mem[];
+    const LdsArgs& F = A.f;
+    uint16_t* blob = reinterpret_cast<uint16_t*>(lds_mem + F.dbl_max);
+    double* seg = A.seg + (size_t)blockIdx.x * A.K * A.ck_n;
+    int cur = -1;
+    for (;;) {
+        int qi = 0;
+        if (threadIdx.x == 0) qi = atomicAdd(A.queue2, 1);
+        qi = __builtin_amdgcn_readfirstlane(__shfl(qi, 0, W));
+        if (qi >= F.n_inst) break;
+        const int inst = __builtin_amdgcn_readfirstlane(F.order[qi]);
+        const int cid = __builtin_amdgcn_readfirstlane(F.inst_cid[inst]);
+        const int rc = __builtin_amdgcn_readfirstlane(F.stats[inst].retcode);
+        if (rc != CS_OK) continue;
+        const CPlan c = load_header(F.cplans + cid);
+        if (cid != cur) {
+            copy_image(c, blob);
+            cur = cid;
+            __syncthreads();
+        }
+        const View v = make_view(lds_mem, blob, c);
+        run_reverse<NR>(A, c, F.plans + cid, F.params + F.p_off[inst], F.u0 + F.u_off[inst], inst,
+                        cid, v, seg);
+    }
+}
+
+}  // namespace
+
+int adj_lds_max_blocks(int nr, size_t lds) {
+    int a = 0, b = 0;
+    switch (nr) {
+        case 1:
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_adj_lds_forward<1>, W, lds);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adj_lds_reverse<1>, W, lds);
+            break;
+        case 2:
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_adj_lds_forward<2>, W, lds);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adj_lds_reverse<2>, W, lds);
+            break;
+        case 4:
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_adj_lds_forward<4>, W, lds);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adj_lds_reverse<4>, W, lds);
+            break;
+        default:
+            return 0;
+    }
+    return a < b ? a : b;
+}
+
+hipError_t launch_adj_lds(int nr, bool reverse, const AdjLdsArgs& a, int nblocks, size_t lds,
+                          hipStream_t s) {
+#define CEDAR_ADJ_LDS_LAUNCH(R)                                                                  \
+    if (reverse) hipLaunchKernelGGL(k_adj_lds_reverse<R>, dim3(nblocks), dim3(W), lds, s, a);      \
+    else hipLaunchKernelGGL(k_adj_lds_forward<R>, dim3(nblocks), dim3(W), lds, s, a);
+    switch (nr) {
+        case 1: CEDAR_ADJ_LDS_LAUNCH(1) break;
+        case 2: CEDAR_ADJ_LDS_LAUNCH(2) break;
+        case 4: CEDAR_ADJ_LDS_LAUNCH(4) break;
+        default: return hipErrorInvalidValue;
+    }
+#undef CEDAR_ADJ_LDS_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace cedar

@@ -125,16 +125,16 @@ struct Packer {
 // keeps compact_ok = false and runs on the global-plan kernel.
 int upload_compact(cs_ctx* ctx, Circuit& C) {
     std::vector<uint16_t> blob;
-    std::vector<int32_t> colnat, diffrow, probe, dev;
+    std::vector<int32_t> colnat, diffrow, probe, dev, devrow;
     const char* why = "";
     CPlan cp{};
     if (C.cbuf) { (void)hipFree(C.cbuf); C.cbuf = nullptr; }
-    C.compact_ok = build_compact(C.plan, blob, cp, colnat, diffrow, probe, dev, &why);
+    C.compact_ok = build_compact(C.plan, blob, cp, colnat, diffrow, probe, dev, devrow, &why);
     C.compact_why = why;
     if (!C.compact_ok) return CS_OK;
     Packer pk;
     size_t o_blob = pk.add(blob), o_cn = pk.add(colnat), o_df = pk.add(diffrow), o_pr = pk.add(probe),
-           o_dev = pk.add(dev);
+           o_dev = pk.add(dev), o_dr = pk.add(devrow);
     HIPCHK(ctx, hipMalloc(&C.cbuf, pk.bytes.size()));
     HIPCHK(ctx, hipMemcpy(C.cbuf, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice));
     char* b = (char*)C.cbuf;
@@ -143,6 +143,7 @@ int upload_compact(cs_ctx* ctx, Circuit& C) {
     cp.diffrow = (const int32_t*)(b + o_df);
     cp.probe = (const int32_t*)(b + o_pr);
     cp.dev = (const int32_t*)(b + o_dev);
+    cp.devrow = (const int32_t*)(b + o_dr);
     C.hcp = cp;
     return CS_OK;
 }
@@ -1092,6 +1093,10 @@ int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes, double* alg_fl
 }
 
 // ---- batched adjoint ------------------------------------------------------------------
+// Runs on the LDS-resident kernels: the instances are grouped into the same
+// size classes as a batched transient (cs_batch_create with the LDS kernel);
+// per class the recording forward and the reverse kernel run back to back on
+// one stream.
 int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const double* params,
                      const double* u0, const cs_tran_opts* o, const cs_adjoint_opts* ao,
                      double* energy, double* grad, int64_t* n_steps, double* t_log,
@@ -1100,131 +1105,111 @@ int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const
         return fail(ctx, CS_E_INVALID, "bad argument");
     if (o->init != CS_INIT_UIC) return fail(ctx, CS_E_INVALID, "the adjoint needs CS_INIT_UIC");
     if (ao->ckpt_every <= 0 || ao->max_steps <= 0) return fail(ctx, CS_E_INVALID, "bad adjoint options");
-#ifndef CEDAR_ENABLE_GLOBAL_PLAN_ADJOINT
-    // The global-plan adjoint kernels (adjoint.hip) are switched off: a GPU
-    // run of them stalled for minutes on a case that finishes in seconds
-    // under a CPU emulation of the same kernel code, and no cause has been
-    // found.  Build with -DCEDAR_ENABLE_GLOBAL_PLAN_ADJOINT to turn them on.
-    return fail(ctx, CS_E_INVALID, "the batched adjoint is switched off in this build");
-#endif
     (void)hipSetDevice(ctx->device);
-    std::vector<void*> allocs;
-    struct Guard {
-        std::vector<void*>& a;
-        ~Guard() { for (void* p : a) (void)hipFree(p); }
-    } guard{allocs};
-    auto dalloc = [&](void** p, size_t bytes) -> int {
-        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-        if (e != hipSuccess) return fail(ctx, CS_E_OOM, "adjoint allocation failed");
-        allocs.push_back(*p);
-        return CS_OK;
-    };
-    std::vector<int64_t> p_off, u_off, g_off;
-    int64_t po = 0, uo = 0, go = 0;
-    int32_t nnz_max = 1, n_max = 1, st_max = 1, lin_max = 1;
-    std::vector<DPlan> plans(ctx->circuits.size());
-    std::vector<int32_t> obj(3 * ctx->circuits.size(), 0);
-    for (size_t c = 0; c < ctx->circuits.size(); c++)
-        if (ctx->circuits[c]) {
-            plans[c] = ctx->circuits[c]->hd;
-            const Plan& P = ctx->circuits[c]->plan;
-            if (ao->vsrc >= 0 && 2 * ao->vsrc + 1 < (int32_t)P.vsrc_nodes.size()) {
-                obj[3 * c] = P.vsrc_nodes[2 * ao->vsrc];
-                obj[3 * c + 1] = P.vsrc_nodes[2 * ao->vsrc + 1];
-                obj[3 * c + 2] = P.n_nodes + ao->vsrc;
-            }
-        }
+    std::vector<int64_t> g_off;
+    int64_t go = 0;
+    int32_t ck_n = 1;
+    std::vector<int32_t> obj(3 * ctx->circuits.size(), (int32_t)CP_GROUND);
     for (int32_t i = 0; i < n_inst; i++) {
         Circuit* C = get_circuit(ctx, inst_cid[i]);
         if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id");
         const Plan& P = C->plan;
         if (!P.uic_ok) return fail(ctx, CS_E_INVALID, "UIC not supported by this circuit");
-        if (2 * ao->vsrc + 1 >= (int32_t)P.vsrc_nodes.size())
+        if (ao->vsrc < 0 || 2 * ao->vsrc + 1 >= (int32_t)P.vsrc_nodes.size())
             return fail(ctx, CS_E_INVALID, "objective source index out of range");
-        p_off.push_back(po); u_off.push_back(uo); g_off.push_back(go);
-        po += P.lay.n_params; uo += P.n; go += P.n_mos;
-        nnz_max = std::max(nnz_max, P.nnz); n_max = std::max(n_max, P.n);
-        st_max = std::max(st_max, P.n_stamp); lin_max = std::max(lin_max, P.n_lin);
+        g_off.push_back(go);
+        go += P.n_mos;
+        ck_n = std::max(ck_n, P.n);
+        const int32_t a = P.vsrc_nodes[2 * ao->vsrc], b = P.vsrc_nodes[2 * ao->vsrc + 1];
+        const size_t c = (size_t)inst_cid[i];
+        obj[3 * c] = a >= 0 ? P.pcol[a] : (int32_t)CP_GROUND;
+        obj[3 * c + 1] = b >= 0 ? P.pcol[b] : (int32_t)CP_GROUND;
+        obj[3 * c + 2] = P.pcol[P.n_nodes + ao->vsrc];
     }
-    if (n_max > 512) return fail(ctx, CS_E_RANGE, "adjoint supports up to 512 unknowns per instance");
-    const int nr = n_max <= 64 ? 1 : n_max <= 128 ? 2 : n_max <= 256 ? 4 : 8;
-    std::vector<int32_t> order(n_inst);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-        return ctx->circuits[inst_cid[a]]->plan.nnz > ctx->circuits[inst_cid[b]]->plan.nnz;
-    });
-    AdjArgs A{};
-    BatchArgs& B = A.b;
+    cs_tran_opts o2 = *o;
+    o2.batch_kernel = CS_BATCH_KERNEL_LDS;
+    o2.n_out = 0;
+    int32_t bid = -1;
+    int rc = cs_batch_create(ctx, n_inst, inst_cid, params, u0, &o2, &bid);
+    if (rc != CS_OK) return rc;
+    std::vector<void*> allocs;
+    struct Guard {
+        cs_ctx* ctx;
+        int32_t bid;
+        std::vector<void*>& a;
+        ~Guard() {
+            for (void* q : a) (void)hipFree(q);
+            (void)cs_batch_destroy(ctx, bid);
+        }
+    } guard{ctx, bid, allocs};
+    Batch* B = get_batch(ctx, bid);
+    auto dalloc = [&](void** q, size_t bytes) -> int {
+        hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) return fail(ctx, CS_E_OOM, "adjoint allocation failed");
+        allocs.push_back(*q);
+        return CS_OK;
+    };
     const int64_t cap = ao->max_steps;
-    const int64_t ckcap = cap / ao->ckpt_every + 1;
-    int rc;
-    void *d_plans, *d_cid, *d_poff, *d_params, *d_uoff, *d_u0, *d_uf, *d_st, *d_order, *d_q, *d_q2,
-        *d_tl, *d_npl, *d_ck, *d_ns, *d_E, *d_g, *d_goff, *d_obj;
-    if ((rc = dalloc(&d_plans, sizeof(DPlan) * plans.size())) ||
-        (rc = dalloc(&d_cid, sizeof(int32_t) * n_inst)) ||
-        (rc = dalloc(&d_poff, sizeof(int64_t) * n_inst)) ||
-        (rc = dalloc(&d_params, sizeof(double) * po)) ||
-        (rc = dalloc(&d_uoff, sizeof(int64_t) * n_inst)) ||
-        (rc = dalloc(&d_u0, sizeof(double) * uo)) || (rc = dalloc(&d_uf, sizeof(double) * uo)) ||
-        (rc = dalloc(&d_st, sizeof(cs_tran_stats) * n_inst)) ||
-        (rc = dalloc(&d_order, sizeof(int32_t) * n_inst)) ||
-        (rc = dalloc(&d_q, sizeof(int32_t))) || (rc = dalloc(&d_q2, sizeof(int32_t))) ||
-        (rc = dalloc(&d_tl, sizeof(double) * cap * n_inst)) ||
+    const int64_t K = ao->ckpt_every;
+    const int64_t ckcap = cap / K + 1;
+    const size_t ncls = B->classes.size();
+    void *d_tl, *d_npl, *d_ck, *d_ns, *d_E, *d_g, *d_goff, *d_obj, *d_q2;
+    if ((rc = dalloc(&d_tl, sizeof(double) * cap * n_inst)) ||
         (rc = dalloc(&d_npl, sizeof(int32_t) * cap * n_inst)) ||
-        (rc = dalloc(&d_ck, sizeof(double) * (size_t)ckcap * 3 * n_max * n_inst)) ||
-        (rc = dalloc(&d_ns, sizeof(int64_t) * n_inst)) ||
-        (rc = dalloc(&d_E, sizeof(double) * n_inst)) || (rc = dalloc(&d_g, sizeof(double) * go)) ||
+        (rc = dalloc(&d_ck, sizeof(double) * (size_t)ckcap * 3 * ck_n * n_inst)) ||
+        (rc = dalloc(&d_ns, sizeof(int64_t) * n_inst)) || (rc = dalloc(&d_E, sizeof(double) * n_inst)) ||
+        (rc = dalloc(&d_g, sizeof(double) * std::max<int64_t>(go, 1))) ||
         (rc = dalloc(&d_goff, sizeof(int64_t) * n_inst)) ||
-        (rc = dalloc(&d_obj, sizeof(int32_t) * obj.size())))
+        (rc = dalloc(&d_obj, sizeof(int32_t) * obj.size())) ||
+        (rc = dalloc(&d_q2, sizeof(int32_t) * ncls)))
         return rc;
-    HIPCHK(ctx, hipMemcpy(d_plans, plans.data(), sizeof(DPlan) * plans.size(), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_cid, inst_cid, sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_poff, p_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_params, params, sizeof(double) * po, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_uoff, u_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_u0, u0, sizeof(double) * uo, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * n_inst, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(d_goff, g_off.data(), sizeof(int64_t) * n_inst, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(d_obj, obj.data(), sizeof(int32_t) * obj.size(), hipMemcpyHostToDevice));
-    B.plans = (const DPlan*)d_plans; B.inst_cid = (const int32_t*)d_cid;
-    B.p_off = (const int64_t*)d_poff; B.params = (const double*)d_params;
-    B.u_off = (const int64_t*)d_uoff; B.u0 = (const double*)d_u0; B.w_off = nullptr;
-    B.wave = nullptr; B.u_final = (double*)d_uf; B.stats = (cs_tran_stats*)d_st;
-    B.order = (const int32_t*)d_order; B.queue = (int32_t*)d_q;
-    B.n_inst = n_inst; B.nnz_max = nnz_max; B.n_max = n_max; B.stamp_max = st_max;
-    B.lin_max = lin_max; B.n_probe_max = 0; B.o = *o;
-    B.o.n_out = 0;
-    A.t_log = (double*)d_tl; A.npts_log = (int32_t*)d_npl; A.cap = cap;
-    A.ckpt = (double*)d_ck; A.ckcap = ckcap; A.K = ao->ckpt_every;
-    A.n_steps = (int64_t*)d_ns; A.energy = (double*)d_E; A.grad = (double*)d_g;
-    A.g_off = (const int64_t*)d_goff; A.obj = (const int32_t*)d_obj; A.queue2 = (int32_t*)d_q2;
-    const size_t lds = adj_lds_bytes(B);
-    const int per_cu = adj_max_blocks(nr, lds);
-    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "adjoint kernels do not fit");
-    const int nblocks = (int)std::min<int64_t>(n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
-    void *d_gl, *d_cl, *d_lb, *d_seg;
-    if ((rc = dalloc(&d_gl, sizeof(double) * (size_t)nblocks * nnz_max)) ||
-        (rc = dalloc(&d_cl, sizeof(double) * (size_t)nblocks * nnz_max)) ||
-        (rc = dalloc(&d_lb, sizeof(double) * (size_t)nblocks * lin_max)) ||
-        (rc = dalloc(&d_seg, sizeof(double) * (size_t)nblocks * ao->ckpt_every * n_max)))
-        return rc;
-    B.glin = (double*)d_gl; B.clin = (double*)d_cl; B.linbuf = (double*)d_lb;
-    A.seg = (double*)d_seg;
+    std::vector<AdjLdsArgs> args(ncls);
+    std::vector<int> nblk(ncls);
+    for (size_t k = 0; k < ncls; k++) {
+        const LdsClass& L = B->classes[k];
+        const int per_cu = adj_lds_max_blocks(L.nr, L.lds);
+        if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "adjoint kernels do not fit on a CU");
+        nblk[k] = (int)std::min<int64_t>(L.n_inst, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+        void* d_seg;
+        if ((rc = dalloc(&d_seg, sizeof(double) * (size_t)nblk[k] * K * ck_n))) return rc;
+        AdjLdsArgs& a = args[k];
+        a.f = L.args;
+        a.t_log = (double*)d_tl;
+        a.npts_log = (int32_t*)d_npl;
+        a.cap = cap;
+        a.ckpt = (double*)d_ck;
+        a.ckcap = ckcap;
+        a.K = (int32_t)K;
+        a.ck_n = ck_n;
+        a.n_steps = (int64_t*)d_ns;
+        a.energy = (double*)d_E;
+        a.grad = (double*)d_g;
+        a.g_off = (const int64_t*)d_goff;
+        a.obj = (const int32_t*)d_obj;
+        a.seg = (double*)d_seg;
+        a.queue2 = (int32_t*)d_q2 + k;
+    }
     hipStream_t s = ctx->stream;
-    hipEvent_t e0, e1;
-    HIPCHK(ctx, hipEventCreate(&e0));
-    HIPCHK(ctx, hipEventCreate(&e1));
-    HIPCHK(ctx, hipMemsetAsync(d_q, 0, sizeof(int32_t), s));
-    HIPCHK(ctx, hipMemsetAsync(d_q2, 0, sizeof(int32_t), s));
-    HIPCHK(ctx, hipEventRecord(e0, s));
-    HIPCHK(ctx, launch_adj(nr, false, A, nblocks, lds, s));
-    HIPCHK(ctx, launch_adj(nr, true, A, nblocks, lds, s));
-    HIPCHK(ctx, hipEventRecord(e1, s));
-    HIPCHK(ctx, hipEventSynchronize(e1));
+    HIPCHK(ctx, hipMemsetAsync(B->d_queues, 0, sizeof(int32_t) * ncls, s));
+    HIPCHK(ctx, hipMemsetAsync(d_q2, 0, sizeof(int32_t) * ncls, s));
+    HIPCHK(ctx, hipEventRecord(B->ev0, s));
+    const size_t ns = B->class_ev.size();
+    for (size_t k = 0; k < ns; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->side[k], B->ev0, 0));
+    for (size_t k = 0; k < ncls; k++) {
+        const LdsClass& L = B->classes[k];
+        HIPCHK(ctx, launch_adj_lds(L.nr, false, args[k], nblk[k], L.lds, ctx->side[k % ns]));
+        HIPCHK(ctx, launch_adj_lds(L.nr, true, args[k], nblk[k], L.lds, ctx->side[k % ns]));
+    }
+    for (size_t k = 0; k < ns; k++) {
+        HIPCHK(ctx, hipEventRecord(B->class_ev[k], ctx->side[k]));
+        HIPCHK(ctx, hipStreamWaitEvent(s, B->class_ev[k], 0));
+    }
+    HIPCHK(ctx, hipEventRecord(B->ev1, s));
+    HIPCHK(ctx, hipEventSynchronize(B->ev1));
     float ms = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(ctx, hipEventElapsedTime(&ms, B->ev0, B->ev1));
     if (kernel_ms) *kernel_ms = ms;
     HIPCHK(ctx, hipMemcpy(energy, d_E, sizeof(double) * n_inst, hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(grad, d_g, sizeof(double) * go, hipMemcpyDeviceToHost));
@@ -1232,9 +1217,7 @@ int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const
     if (t_log) HIPCHK(ctx, hipMemcpy(t_log, d_tl, sizeof(double) * cap * n_inst, hipMemcpyDeviceToHost));
     if (npts_log)
         HIPCHK(ctx, hipMemcpy(npts_log, d_npl, sizeof(int32_t) * cap * n_inst, hipMemcpyDeviceToHost));
-    std::vector<cs_tran_stats> sts(n_inst);
-    HIPCHK(ctx, hipMemcpy(sts.data(), d_st, sizeof(cs_tran_stats) * n_inst, hipMemcpyDeviceToHost));
-    if (st) std::memcpy(st, sts.data(), sizeof(cs_tran_stats) * n_inst);
+    if (st) HIPCHK(ctx, hipMemcpy(st, B->args.stats, sizeof(cs_tran_stats) * n_inst, hipMemcpyDeviceToHost));
     return CS_OK;
 }
 

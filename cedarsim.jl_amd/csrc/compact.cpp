@@ -37,7 +37,8 @@ struct Blob {
 
 bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
                    std::vector<int32_t>& colnat, std::vector<int32_t>& diffrow,
-                   std::vector<int32_t>& probe, std::vector<int32_t>& dev, const char** why) {
+                   std::vector<int32_t>& probe, std::vector<int32_t>& dev,
+                   std::vector<int32_t>& devrow, const char** why) {
     const int32_t n = P.n, nnz = P.nnz, nm = P.n_mos, ns = P.n_src;
     cp = CPlan{};
     *why = "";
@@ -157,6 +158,51 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
         }
         bxt[l + 1] = (int32_t)xtask.size() / 4;
     }
+    // transposed solves (adjoint): U^T z = b over U's columns, ascending
+    // (scaled by 1/U_kk), then L^T w = z over L's columns, descending
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> ucol(n), lcolv(n);   // (row, pos)
+    for (int32_t i = 0; i < n; i++)
+        for (int32_t e = P.rowptr[i]; e < P.rowptr[i + 1]; e++) {
+            const int32_t j = P.colperm[e];
+            if (j > i) ucol[j].push_back({i, e});
+            else if (j < i) lcolv[j].push_back({i, e});
+        }
+    std::vector<int32_t> tu(n, -1), tl(n, -1);
+    int32_t nTU = 0, nTL = 0;
+    for (int32_t k = 0; k < n; k++) {
+        int32_t mx = -1;
+        for (auto& ie : ucol[k]) mx = std::max(mx, tu[ie.first]);
+        tu[k] = mx + 1;
+        nTU = std::max(nTU, tu[k] + 1);
+    }
+    for (int32_t k = n - 1; k >= 0; k--) {
+        if (lcolv[k].empty()) continue;
+        int32_t mx = -1;
+        for (auto& ie : lcolv[k]) mx = std::max(mx, tl[ie.first]);
+        tl[k] = mx + 1;
+        nTL = std::max(nTL, tl[k] + 1);
+    }
+    std::vector<int32_t> tudesc(2 * (size_t)nTU), tldesc(2 * (size_t)nTL);
+    for (int32_t l = 0; l < nTU; l++) {
+        tudesc[2 * l] = (int32_t)xtask.size() / 4;
+        for (int32_t k = 0; k < n; k++) {
+            if (tu[k] != l) continue;
+            const int32_t ub = (int32_t)upd.size() / 2;
+            for (auto& ie : ucol[k]) push_upd(ie.second, ie.first);
+            xtask.insert(xtask.end(), {k, ub, (int32_t)upd.size() / 2, 1});
+        }
+        tudesc[2 * l + 1] = (int32_t)xtask.size() / 4;
+    }
+    for (int32_t l = 0; l < nTL; l++) {
+        tldesc[2 * l] = (int32_t)xtask.size() / 4;
+        for (int32_t k = 0; k < n; k++) {
+            if (tl[k] != l) continue;
+            const int32_t ub = (int32_t)upd.size() / 2;
+            for (auto& ie : lcolv[k]) push_upd(ie.second, ie.first);
+            xtask.insert(xtask.end(), {k, ub, (int32_t)upd.size() / 2, 0});
+        }
+        tldesc[2 * l + 1] = (int32_t)xtask.size() / 4;
+    }
     if (upd.size() / 2 >= 0xFFFF) {
         *why = "too many LU updates for 16-bit indices";
         return false;
@@ -199,6 +245,28 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
     cp.o_ftask = b.add(ftask, 4);
     cp.o_xtask = b.add(xtask, 4);
     cp.o_upd = b.add(upd, 2);
+    // linear entries by column
+    std::vector<int32_t> cdesc(2 * (size_t)n), lcl, lrow;
+    {
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> bycol(n);   // (lin index, row)
+        for (int32_t k = 0; k < n; k++)
+            for (int32_t l = lptr[k]; l < lptr[k + 1]; l++) bycol[lcol[l]].push_back({l, k});
+        for (int32_t j = 0; j < n; j++) {
+            cdesc[2 * j] = (int32_t)lcl.size();
+            for (auto& lr : bycol[j]) {
+                lcl.push_back(lr.first);
+                lrow.push_back(lr.second);
+            }
+            cdesc[2 * j + 1] = (int32_t)lcl.size();
+        }
+    }
+    cp.o_tudesc = b.add(tudesc, 2);
+    cp.o_tldesc = b.add(tldesc, 2);
+    cp.o_cdesc = b.add(cdesc, 2);
+    cp.o_lcl = b.add(lcl);
+    cp.o_lrow = b.add(lrow);
+    cp.nTU = nTU;
+    cp.nTL = nTL;
     b.align(8);
     if (!b.ok) {
         *why = "plan index does not fit 16 bits";
@@ -236,6 +304,12 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
             dev[4 * m + q] = v >= 0 ? P.pcol[v] : (int32_t)CP_GROUND;
         }
         dev[4 * m + 3] = P.mos[4 * m + 3];
+    }
+    devrow.assign(2 * (size_t)nm, 0);
+    for (int32_t m = 0; m < nm; m++) {
+        const int32_t d = P.mos[4 * m], s = P.mos[4 * m + 2];
+        devrow[2 * m] = d >= 0 ? P.prow[d] : (int32_t)CP_GROUND;
+        devrow[2 * m + 1] = s >= 0 ? P.prow[s] : (int32_t)CP_GROUND;
     }
     return true;
 }

@@ -16,6 +16,12 @@
 //   lpos[] lcol[]                   linear entries: F position, column
 //   adesc[4*nA]  per A level: F tasks [fb, fe), X tasks [xb, xe)
 //   bdesc[2*nB]  per B level: X tasks [xb, xe)
+//   tudesc[2*nTU], tldesc[2*nTL]   levels of the transposed solves U^T z = b
+//                (scaled X tasks over U's columns) and L^T w = z (X tasks
+//                over L's columns), used by the adjoint
+//   cdesc[2*n] lcl[] lrow[]         linear entries by column: for column j
+//                the range [c0, c1) of lcl (linear-entry index) and lrow (its
+//                row), so C^T v is a gather
 //   src[4*n_src]                    kind, offset in wd, length, 0
 //   ftask[4*]  (e, ub, ue, ctl)     F[e] -= sum F[a]F[b] over upd[ub..ue);
 //                                   ctl < 0x7FFF: then *= inv[ctl] (L entry);
@@ -48,13 +54,15 @@ struct CPlan {
     int32_t dbl;          // LDS doubles of one instance
     int32_t o_rdesc, o_jptr, o_jref, o_rref, o_lpos, o_lcol;
     int32_t o_adesc, o_bdesc, o_src, o_ftask, o_xtask, o_upd;
-    int32_t n_ftask, n_xtask, n_upd, pad;
+    int32_t o_tudesc, o_tldesc, o_cdesc, o_lcl, o_lrow, nTU, nTL;
+    int32_t n_ftask, n_xtask, n_upd;
     // global arrays, read when an instance starts
     const uint16_t* blob;
     const int32_t* colnat;    // [n] natural unknown of pivot column k
     const int32_t* diffrow;   // [n] pivot row k carries charge
     const int32_t* probe;     // [n] probe slot of pivot column k, or -1
     const int32_t* dev;       // [4*n_mos] d, g, s as pivot columns (0xFFFF ground), type
+    const int32_t* devrow;    // [2*n_mos] pivot rows of d and s (0xFFFF ground)
 };
 
 struct Plan;
@@ -62,6 +70,7 @@ struct Plan;
 // 16-bit index would overflow; the caller then uses the global-plan kernel.
 bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
                    std::vector<int32_t>& colnat, std::vector<int32_t>& diffrow,
-                   std::vector<int32_t>& probe, std::vector<int32_t>& dev, const char** why);
+                   std::vector<int32_t>& probe, std::vector<int32_t>& dev,
+                   std::vector<int32_t>& devrow, const char** why);
 
 }  // namespace cedar

@@ -66,23 +66,25 @@ struct LdsArgs {
     cs_tran_opts o;
 };
 
-// Batched discrete adjoint (adjoint.hip).
-struct AdjArgs {
-    BatchArgs b;
+// Batched discrete adjoint on the LDS-resident kernel (adj_lds.hip): one
+// launch per size class for the recording forward pass, one for the reverse.
+struct AdjLdsArgs {
+    LdsArgs f;              // the class's transient arguments (queue: forward)
     double* t_log;          // [n_inst][cap]
     int32_t* npts_log;      // [n_inst][cap]
     int64_t cap;
-    double* ckpt;           // [n_inst][ckcap][3][n_max]
+    double* ckpt;           // [n_inst][ckcap][3][ck_n] pivot-order states
     int64_t ckcap;
-    int32_t K;
-    int32_t pad;
+    int32_t K;              // checkpoint interval (accepted steps)
+    int32_t ck_n;           // doubles per checkpointed state (max unknowns)
     int64_t* n_steps;       // [n_inst]
     double* energy;         // [n_inst]
     double* grad;           // per instance at g_off, n_mos entries
     const int64_t* g_off;
-    const int32_t* obj;     // per circuit: a, b, j (natural indices)
-    double* seg;            // per workgroup: K * n_max
-    int32_t* queue2;
+    const int32_t* obj;     // per circuit: a, b, j as pivot columns (0xFFFF ground)
+    double* seg;            // per workgroup: K * ck_n recomputed states
+    int32_t* queue2;        // the class's reverse queue head
+    int32_t pad;
 };
 
 constexpr int kSingleThreads = 256;
@@ -108,10 +110,10 @@ size_t lds_kernel_bytes(int32_t dbl_max, int32_t words_max);
 int lds_max_blocks(int nr, size_t lds);
 hipError_t launch_tran_lds(int nr, const LdsArgs& a, int nblocks, size_t lds, hipStream_t s);
 
-// batched adjoint: LDS bytes, resident workgroups per CU (min over the two
-// kernels), launch of the forward (recording) or reverse kernel
-size_t adj_lds_bytes(const BatchArgs& b);
-int adj_max_blocks(int nr, size_t lds);
-hipError_t launch_adj(int nr, bool reverse, const AdjArgs& a, int nblocks, size_t lds, hipStream_t s);
+// LDS-resident adjoint: resident workgroups per CU (min over both kernels),
+// launch of the recording forward or the reverse kernel
+int adj_lds_max_blocks(int nr, size_t lds);
+hipError_t launch_adj_lds(int nr, bool reverse, const AdjLdsArgs& a, int nblocks, size_t lds,
+                          hipStream_t s);
 
 }  // namespace cedar

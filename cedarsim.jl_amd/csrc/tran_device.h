@@ -1,11 +1,11 @@
-// tran_device.h — the per-instance transient loop of the batched kernels,
-// shared by the plain batched transient (kernels.hip) and the adjoint's
-// recording forward pass (adjoint.hip).  One workgroup of NT = 64 threads
-// (one wavefront) owns one circuit instance; lane l owns unknowns
-// l, l+64, ... (NR of them) and keeps their history in registers.
+// tran_device.h — the per-instance transient loop of the global-plan batched
+// kernel (k_tran_batch, kernels.hip), which reads the plan from global
+// memory in natural unknown order.  One workgroup of NT = 64 threads (one
+// wavefront) owns one circuit instance; lane l owns unknowns l, l+64, ...
+// (NR of them) and keeps their history in registers.
 //
 // Rec is a recording policy called on every accepted point (including the
-// initial one, as step 0); the plain batch passes NoRec.
+// initial one, as step 0); the batch passes NoRec.
 #pragma once
 #include "device.h"
 #include "kernels.h"

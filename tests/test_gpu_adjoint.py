@@ -4,7 +4,8 @@ The GPU records its accepted-step sequence; the oracle replays exactly that
 sequence (same discrete scheme) for (a) its own adjoint and (b) central finite
 differences in each width (h = 1e-6 relative).  Bars: energy to 1e-9
 relative, gradients to 1e-6 against the oracle adjoint and 1e-5 against the
-finite differences.
+finite differences.  Runs are bounded (max_steps counts accepted plus
+rejected steps) so that a failing case ends with an error code.
 """
 import numpy as np
 import pytest
@@ -13,12 +14,7 @@ import cedarhip as ch
 from cedarhip.workloads import ring_initial_state, ring_oscillator
 from oracle import oracle as O
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the batched adjoint kernels are switched off: a GPU run of this test "
-                            "stalled (killed after 180 s without output) while the same kernel "
-                            "code finishes in seconds under CPU emulation; no cause found yet"),
-]
+pytestmark = pytest.mark.gpu
 
 T_STOP = 0.3e-9
 
@@ -44,9 +40,12 @@ def _batch(sizes, seed):
     return flats, params, u0s
 
 
+NTOL = 1e-8
+
+
 def _opts():
     return ch.TranOptions(t_start=0.0, t_stop=T_STOP, reltol=1e-6, abstol=1e-9,
-                          newton_tol=1e-9, newton_maxit=30, init=1)
+                          newton_tol=NTOL, newton_maxit=30, init=1, max_steps=60_000)
 
 
 def test_adjoint_matches_oracle_and_fd(engine):
@@ -54,9 +53,10 @@ def test_adjoint_matches_oracle_and_fd(engine):
     flats, params, u0s = _batch([5, 7, 11, 5], seed=4)
     o = _opts()
     res = ch.adjoint_batch(flats, params, u0s, o, vsrc=0, ckpt_every=64, keep_logs=True,
-                           engine=engine)
+                           max_steps=40_000, engine=engine)
+    _say(f"adjoint done: {res.retcodes}, steps {list(res.n_steps)}, {res.kernel_ms:.1f} ms")
     assert all(r == "Success" for r in res.retcodes), res.retcodes
-    oo = dict(t_start=0.0, t_stop=T_STOP, reltol=1e-6, abstol=1e-9, newton_tol=1e-9,
+    oo = dict(t_start=0.0, t_stop=T_STOP, reltol=1e-6, abstol=1e-9, newton_tol=NTOL,
               newton_maxit=30, init=1)
     for i, f in enumerate(flats):
         tl, npl = res.t_logs[i], res.npts_logs[i]
@@ -81,11 +81,14 @@ def test_adjoint_checkpoint_interval_and_determinism(engine):
     _say("adjoint checkpoint interval 64 vs 7, repeat")
     flats, params, u0s = _batch([7, 9], seed=8)
     o = _opts()
-    r64 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, engine=engine)
-    r7 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=7, engine=engine)
-    r64b = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, engine=engine)
+    r64 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, max_steps=40_000, engine=engine)
+    r7 = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=7, max_steps=40_000, engine=engine)
+    r64b = ch.adjoint_batch(flats, params, u0s, o, ckpt_every=64, max_steps=40_000, engine=engine)
+    _say(f"adjoint runs done: {r64.kernel_ms:.1f} / {r7.kernel_ms:.1f} / {r64b.kernel_ms:.1f} ms")
     for i in range(len(flats)):
         assert np.array_equal(r64.grads[i], r64b.grads[i]), "adjoint is not bit-reproducible"
         assert r64.energy[i] == r64b.energy[i]
-        np.testing.assert_allclose(r7.grads[i], r64.grads[i], rtol=1e-10,
-                                   atol=1e-12 * np.max(np.abs(r64.grads[i])))
+        # recomputation from a checkpoint replays the same Newton iterates, so
+        # the checkpoint interval does not change a bit
+        assert np.array_equal(r7.grads[i], r64.grads[i])
+        assert r7.energy[i] == r64.energy[i]
