@@ -93,9 +93,9 @@ __global__ __launch_bounds__(W) void k_adj_lds_forward(AdjLdsArgs A) {
         const int cid = __builtin_amdgcn_readfirstlane(F.inst_cid[inst]);
         const CPlan c = load_header(F.cplans + cid);
         if (cid != cur) {
-            copy_image(c, blob);
+            copy_image(c, blob, lane_id(), W);
             cur = cid;
-            __syncthreads();
+            lds_sync();
         }
         const View v = make_view(lds_mem, blob, c);
         AdjRec rec;
@@ -173,7 +173,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
             flag[r] = c.diffrow[k] ? 1u : 0u;
         }
     }
-    __syncthreads();
+    lds_sync();
     long long nit = 0;
 
     for (long long s = M / K; s >= 0; s--) {
@@ -181,7 +181,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
         const long long m1 = (m0 + K < M + 1) ? m0 + K : M + 1;
         double U0[NR], U1[NR], U2[NR], Q0[NR], Q1[NR], qx[NR];
         const double* cp = ck + (size_t)s * 3 * ck_n;
-        __syncthreads();   // the previous segment's last step still reads u
+        lds_sync();   // the previous segment's last step still reads u
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int k = tid + W * r;
@@ -194,19 +194,19 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
                 v.u[k] = U1[r];
             }
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int k = tid + W * r;
             if (k < n) Q1[r] = row_charge(c, v, k);
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int k = tid + W * r;
             if (k < n) v.u[k] = U0[r];
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int k = tid + W * r;
@@ -222,7 +222,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
             bdf_coeffs(k, tn, T0, T1, a0, a1, a2);
             const int np = npts < k + 1 ? npts : k + 1;
             pred_weights(np, tn, T0, T1, T2, w0, w1, w2);
-            __syncthreads();
+            lds_sync();
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const int kk = tid + W * r;
@@ -235,7 +235,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
                 }
             }
             eval_sources(c, v, tn, CS_MODE_TRAN);
-            __syncthreads();
+            lds_sync();
             newton<NR, NM>(c, v, dr, gmin, a0, qx, pinv, flag, o.newton_maxit, o.newton_tol,
                            o.reltol, o.abstol, nit);
 #pragma unroll
@@ -250,7 +250,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
         }
         // ---- backward over the segment ------------------------------------
         for (long long m = m1 - 1; m >= m0; m--) {
-            __syncthreads();
+            lds_sync();
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const int k = tid + W * r;
@@ -266,14 +266,14 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
             double am0 = 0.0;
             if (m > 0) step_coeffs(tl, npl, m, o.max_order, am0, b1, b2);
             eval_sources(c, v, tm, CS_MODE_TRAN);
-            __syncthreads();
+            lds_sync();
             eval_mos<NM>(c, v, dr, gmin);
-            __syncthreads();
+            lds_sync();
             double qz[NR];
 #pragma unroll
             for (int r = 0; r < NR; r++) qz[r] = 0.0;
             assemble<NR>(c, v, am0, qz, m == 0, pinv, flag);
-            __syncthreads();
+            lds_sync();
             factor<false>(c, v);
             const double ua = oa != CP_GROUND ? v.u[oa] : 0.0, ub = ob != CP_GROUND ? v.u[ob] : 0.0;
             const double uj = v.u[oj];
@@ -288,7 +288,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
                     v.x[k] = -w * dP - c1 * ctl1[r] - c2 * ctl2[r];
                 }
             }
-            __syncthreads();
+            lds_sync();
             solve_transposed(c, v);
             // C^T lambda for this lane's columns
             double ctl[NR];
@@ -324,7 +324,7 @@ __device__ void run_reverse(const AdjLdsArgs& A, const CPlan& c, const DPlan* __
         const int md = tid + W * jd;
         if (md < c.n_mos) grad[md] = gacc[jd];
     }
-    __syncthreads();
+    lds_sync();
 }
 
 template <int NR>
@@ -345,9 +345,9 @@ __global__ __launch_bounds__(W) void k_adj_lds_reverse(AdjLdsArgs A) {
         if (rc != CS_OK) continue;
         const CPlan c = load_header(F.cplans + cid);
         if (cid != cur) {
-            copy_image(c, blob);
+            copy_image(c, blob, lane_id(), W);
             cur = cid;
-            __syncthreads();
+            lds_sync();
         }
         const View v = make_view(lds_mem, blob, c);
         run_reverse<NR>(A, c, F.plans + cid, F.params + F.p_off[inst], F.u0 + F.u_off[inst], inst,

@@ -48,6 +48,7 @@ struct Circuit {
 };
 
 constexpr size_t kLdsMaxBytes = 64 * 1024;   // dynamic LDS of one workgroup
+constexpr size_t kPackLds = 80 * 1024 - 256; // packed launch: two workgroups per CU
 constexpr size_t kSideStreams = 4;           // hardware queues of one process
 
 // One launch of the LDS-resident kernel: the instances of circuits whose
@@ -74,12 +75,15 @@ struct Batch {
     std::vector<LdsClass> classes;             // LDS-resident kernel: one launch each
     std::vector<hipEvent_t> class_ev;
     int32_t* d_queues = nullptr;
+    LdsClass pack;                             // the packed launch of the transient
 };
 
 }  // namespace
 
 struct cs_ctx {
     int device = 0;
+    std::vector<char> bsim_card, bsim_inst;   // host copies (generated structs)
+    bool bsim_ready = false;
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> side;   // default-priority streams for concurrent class launches
     int n_cu = 0;
@@ -836,6 +840,63 @@ static int build_lds_classes(cs_ctx* ctx, Batch& B, const std::vector<int32_t>& 
     return CS_OK;
 }
 
+// The packed launch: chunks of up to kPackWaves same-circuit instances, the
+// largest circuits first; a workgroup holds one plan image and one LDS region
+// of doubles per wavefront.  Returns CS_E_RANGE when a circuit does not fit
+// even one wavefront beside its image.
+static int build_pack(cs_ctx* ctx, Batch& B, const std::vector<int32_t>& used,
+                      const std::vector<int32_t>& c_nr,
+                      const std::function<int(void**, size_t)>& dalloc) {
+    std::vector<int32_t> cs = used;
+    std::stable_sort(cs.begin(), cs.end(), [&](int32_t a, int32_t b) {
+        return ctx->circuits[a]->hcp.n > ctx->circuits[b]->hcp.n;
+    });
+    std::vector<int32_t> kmax(ctx->circuits.size(), 0);
+    size_t need = 0;
+    int32_t nr = 1;
+    for (int32_t c : cs) {
+        const CPlan& cp = ctx->circuits[c]->hcp;
+        const size_t img = 2 * (size_t)cp.words, dblb = 8 * (size_t)((cp.dbl + 1) & ~1);
+        if (img + dblb > kPackLds) return CS_E_RANGE;
+        kmax[c] = (int32_t)std::min<size_t>(kPackWaves, (kPackLds - img) / dblb);
+        need = std::max(need, img + kmax[c] * dblb);
+        nr = std::max(nr, c_nr[c]);
+    }
+    std::vector<std::vector<int32_t>> per(ctx->circuits.size());
+    for (int32_t i = 0; i < B.n_inst; i++) per[B.cid[i]].push_back(i);
+    std::vector<int32_t> order;
+    std::vector<int4> chunks;
+    for (int32_t c : cs)
+        for (size_t q = 0; q < per[c].size(); q += kmax[c]) {
+            const int32_t cnt = (int32_t)std::min<size_t>(kmax[c], per[c].size() - q);
+            chunks.push_back(make_int4(c, (int32_t)order.size(), cnt, 0));
+            order.insert(order.end(), per[c].begin() + q, per[c].begin() + q + cnt);
+        }
+    void *d_order, *d_chunks, *d_q;
+    int rc;
+    if ((rc = dalloc(&d_order, sizeof(int32_t) * order.size())) ||
+        (rc = dalloc(&d_chunks, sizeof(int4) * chunks.size())) || (rc = dalloc(&d_q, sizeof(int32_t))))
+        return rc;
+    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice));
+    LdsClass& L = B.pack;
+    L.nr = nr;
+    L.lds = need;
+    L.n_inst = B.n_inst;
+    const int per_cu = pack_max_blocks(nr, need);
+    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "packed LDS kernel does not fit on a CU");
+    L.nblocks = (int32_t)std::min<int64_t>((int64_t)chunks.size(), (int64_t)per_cu * std::max(ctx->n_cu, 1));
+    LdsArgs& a = L.args;
+    a = B.classes.empty() ? LdsArgs{} : B.classes[0].args;
+    a.order = (const int32_t*)d_order;
+    a.queue = (int32_t*)d_q;
+    a.n_inst = B.n_inst;
+    a.dbl_max = 0;
+    a.chunks = (const int4*)d_chunks;
+    a.n_chunks = (int32_t)chunks.size();
+    return CS_OK;
+}
+
 // ---- batched transient --------------------------------------------------------------
 int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const double* params,
                     const double* u0, const cs_tran_opts* o, int32_t* bid) {
@@ -981,8 +1042,10 @@ int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const 
         A.glin = (double*)d_g;
         A.clin = (double*)d_c;
         A.linbuf = (double*)d_l;
-    } else if ((rc = build_lds_classes(ctx, *B, used, c_nr, c_need, dalloc)) != CS_OK) {
-        return rc;
+    } else {
+        if ((rc = build_lds_classes(ctx, *B, used, c_nr, c_need, dalloc)) != CS_OK) return rc;
+        if ((rc = build_pack(ctx, *B, used, c_nr, dalloc)) != CS_OK)
+            return rc == CS_E_RANGE ? fail(ctx, rc, "a circuit does not fit the packed LDS kernel") : rc;
     }
     HIPCHK(ctx, hipEventCreate(&B->ev0));
     HIPCHK(ctx, hipEventCreate(&B->ev1));
@@ -1007,20 +1070,10 @@ int cs_batch_run(cs_ctx* ctx, int32_t bid, double* kernel_ms) {
         HIPCHK(ctx, launch_tran_batch(B->nr, B->args, B->nblocks, B->lds, s));
         HIPCHK(ctx, hipEventRecord(B->ev1, s));
     } else {
-        // classes largest first, spread over the side streams so that a
-        // class's tail overlaps the next class's start
-        HIPCHK(ctx, hipMemsetAsync(B->d_queues, 0, sizeof(int32_t) * B->classes.size(), s));
+        const LdsClass& L = B->pack;
+        HIPCHK(ctx, hipMemsetAsync(L.args.queue, 0, sizeof(int32_t), s));
         HIPCHK(ctx, hipEventRecord(B->ev0, s));
-        const size_t ns = B->class_ev.size();
-        for (size_t k = 0; k < ns; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->side[k], B->ev0, 0));
-        for (size_t k = 0; k < B->classes.size(); k++) {
-            const LdsClass& L = B->classes[k];
-            HIPCHK(ctx, launch_tran_lds(L.nr, L.args, L.nblocks, L.lds, ctx->side[k % ns]));
-        }
-        for (size_t k = 0; k < ns; k++) {
-            HIPCHK(ctx, hipEventRecord(B->class_ev[k], ctx->side[k]));
-            HIPCHK(ctx, hipStreamWaitEvent(s, B->class_ev[k], 0));
-        }
+        HIPCHK(ctx, launch_tran_pack(L.nr, L.args, L.nblocks, L.lds, s));
         HIPCHK(ctx, hipEventRecord(B->ev1, s));
     }
     HIPCHK(ctx, hipEventSynchronize(B->ev1));
@@ -1061,7 +1114,7 @@ int cs_batch_kernel(cs_ctx* ctx, int32_t bid, int32_t* kernel, int32_t* n_launch
     Batch* B = get_batch(ctx, bid);
     if (!B) return fail(ctx, CS_E_INVALID, "bad batch id");
     if (kernel) *kernel = B->kernel;
-    if (n_launches) *n_launches = B->kernel == CS_BATCH_KERNEL_LDS ? (int32_t)B->classes.size() : 1;
+    if (n_launches) *n_launches = 1;
     return CS_OK;
 }
 
@@ -1091,6 +1144,97 @@ int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes, double* alg_fl
     if (alg_flops) *alg_flops = flops;
     return CS_OK;
 }
+
+// ---- BSIM-CMG device evaluation ---------------------------------------------------------
+#ifdef CEDAR_HAVE_BSIMCMG
+int cs_bsimcmg_available(void) { return 1; }
+
+static void bsim_init(cs_ctx* ctx) {
+    if (ctx->bsim_ready) return;
+    ctx->bsim_card.assign(bsim_card_size(), 0);
+    ctx->bsim_inst.assign(bsim_inst_size(), 0);
+    ctx->bsim_ready = true;
+}
+
+int cs_bsimcmg_param(cs_ctx* ctx, const char* name, double value) {
+    if (!ctx || !name) return fail(ctx, CS_E_INVALID, "bad argument");
+    bsim_init(ctx);
+    if (bsim_set_param(ctx->bsim_card.data(), ctx->bsim_inst.data(), name, value) != 0)
+        return fail(ctx, CS_E_INVALID, std::string("unknown BSIM-CMG parameter ") + name);
+    return CS_OK;
+}
+
+int cs_bsimcmg_eval(cs_ctx* ctx, int64_t n, const double* L, const double* nfin, const double* v,
+                    double temp_c, double* out, int32_t* iters, double* kernel_ms) {
+    if (!ctx || n <= 0 || !L || !nfin || !v || !out) return fail(ctx, CS_E_INVALID, "bad argument");
+    (void)hipSetDevice(ctx->device);
+    bsim_init(ctx);
+    std::vector<char> card = ctx->bsim_card;   // given flags as set; defaults filled here
+    bsim_card_defaults(card.data());
+    std::vector<void*> allocs;
+    struct Guard {
+        std::vector<void*>& a;
+        ~Guard() { for (void* q : a) (void)hipFree(q); }
+    } guard{allocs};
+    auto dalloc = [&](void** q, size_t bytes) -> int {
+        hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) return fail(ctx, CS_E_OOM, "BSIM-CMG allocation failed");
+        allocs.push_back(*q);
+        return CS_OK;
+    };
+    const int per = bsim_out_per();
+    void *d_card, *d_inst, *d_L, *d_nf, *d_v, *d_out, *d_it = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_card, card.size())) || (rc = dalloc(&d_inst, ctx->bsim_inst.size())) ||
+        (rc = dalloc(&d_L, sizeof(double) * n)) || (rc = dalloc(&d_nf, sizeof(double) * n)) ||
+        (rc = dalloc(&d_v, sizeof(double) * 4 * n)) || (rc = dalloc(&d_out, sizeof(double) * per * n)) ||
+        (iters && (rc = dalloc(&d_it, sizeof(int32_t) * n))))
+        return rc;
+    HIPCHK(ctx, hipMemcpy(d_card, card.data(), card.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_inst, ctx->bsim_inst.data(), ctx->bsim_inst.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_L, L, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_nf, nfin, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_v, v, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+    BsimArgs a{};
+    a.card = d_card;
+    a.inst = d_inst;
+    a.n = n;
+    a.L = (const double*)d_L;
+    a.nfin = (const double*)d_nf;
+    a.v = (const double*)d_v;
+    a.temp = temp_c + 273.15;
+    a.out = (double*)d_out;
+    a.iters = (int32_t*)d_it;
+    const int per_cu = bsim_max_blocks();
+    if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "BSIM-CMG kernel does not fit on a CU");
+    const int64_t want = (n + 255) / 256;
+    const int nblocks = (int)std::min<int64_t>(want, (int64_t)per_cu * std::max(ctx->n_cu, 1) * 4);
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+    HIPCHK(ctx, launch_bsimcmg(a, nblocks, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    HIPCHK(ctx, hipMemcpy(out, d_out, sizeof(double) * per * n, hipMemcpyDeviceToHost));
+    if (iters) HIPCHK(ctx, hipMemcpy(iters, d_it, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return CS_OK;
+}
+#else
+int cs_bsimcmg_available(void) { return 0; }
+int cs_bsimcmg_param(cs_ctx* ctx, const char*, double) {
+    return fail(ctx, CS_E_INVALID, "this library was built without the BSIM-CMG model");
+}
+int cs_bsimcmg_eval(cs_ctx* ctx, int64_t, const double*, const double*, const double*, double, double*,
+                    int32_t*, double*) {
+    return fail(ctx, CS_E_INVALID, "this library was built without the BSIM-CMG model");
+}
+#endif
 
 // ---- batched adjoint ------------------------------------------------------------------
 // Runs on the LDS-resident kernels: the instances are grouped into the same

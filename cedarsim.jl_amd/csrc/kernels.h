@@ -64,6 +64,10 @@ struct LdsArgs {
     int32_t n_inst;          // instances in this class
     int32_t dbl_max;         // LDS doubles before the plan image
     cs_tran_opts o;
+    // packed launch: chunks of same-circuit instances, one wavefront each
+    const int4* chunks;      // (circuit, first index into order, count, 0)
+    int32_t n_chunks;
+    int32_t pad_;
 };
 
 // Batched discrete adjoint on the LDS-resident kernel (adj_lds.hip): one
@@ -85,6 +89,19 @@ struct AdjLdsArgs {
     double* seg;            // per workgroup: K * ck_n recomputed states
     int32_t* queue2;        // the class's reverse queue head
     int32_t pad;
+};
+
+// Batched BSIM-CMG evaluation (bsimcmg.hip).
+struct BsimArgs {
+    const void* card;       // the model card (generated Card struct), device
+    const void* inst;       // instance-parameter template (generated Inst), device
+    int64_t n;
+    const double* L;        // [n] gate length
+    const double* nfin;     // [n] fins per finger
+    const double* v;        // [n][4] terminal voltages d, g, s, e
+    double temp;            // kelvin
+    double* out;            // [n][40]: I[4], Q[4], G[16], C[16]
+    int32_t* iters;         // [n] internal-node Newton iterations (may be null)
 };
 
 constexpr int kSingleThreads = 256;
@@ -109,11 +126,25 @@ hipError_t launch_tran_batch(int nr, const BatchArgs& a, int nblocks, size_t lds
 size_t lds_kernel_bytes(int32_t dbl_max, int32_t words_max);
 int lds_max_blocks(int nr, size_t lds);
 hipError_t launch_tran_lds(int nr, const LdsArgs& a, int nblocks, size_t lds, hipStream_t s);
+// packed launch: workgroups of kPackWaves wavefronts, each workgroup runs one
+// chunk of same-circuit instances at a time sharing one plan image in LDS
+constexpr int kPackWaves = 8;
+int pack_max_blocks(int nr, size_t lds);
+hipError_t launch_tran_pack(int nr, const LdsArgs& a, int nblocks, size_t lds, hipStream_t s);
 
 // LDS-resident adjoint: resident workgroups per CU (min over both kernels),
 // launch of the recording forward or the reverse kernel
 int adj_lds_max_blocks(int nr, size_t lds);
 hipError_t launch_adj_lds(int nr, bool reverse, const AdjLdsArgs& a, int nblocks, size_t lds,
                           hipStream_t s);
+
+// BSIM-CMG: sizes of the generated structs, host helpers, launch
+int bsim_card_size();
+int bsim_inst_size();
+int bsim_out_per();
+void bsim_card_defaults(void* card);
+int bsim_set_param(void* card, void* inst, const char* name, double value);
+int bsim_max_blocks();
+hipError_t launch_bsimcmg(const BsimArgs& a, int nblocks, hipStream_t s);
 
 }  // namespace cedar

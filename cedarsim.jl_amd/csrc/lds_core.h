@@ -43,7 +43,18 @@ static __device__ __noinline__ double cold_next_bp(int kind, const double* d, in
 static __device__ __noinline__ double cold_reject(double err, int k) { return reject_factor(err, k); }
 static __device__ __noinline__ double cold_accept(double err, int k) { return accept_factor(err, k); }
 
-__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
+
+// Synchronisation of the instance's own wavefront.  An instance is owned by
+// one wavefront, so its phases need only wavefront-scope ordering of LDS
+// accesses (other wavefronts of the workgroup own other instances): a fence
+// that keeps the compiler from moving memory accesses across, and the
+// wavefront barrier.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ double sv(const double* st, uint32_t r) {
     const double v = st[r >> 1];
@@ -76,11 +87,12 @@ __device__ __forceinline__ CPlan load_header(const CPlan* __restrict__ cp) {
 }
 
 // Copy the plan image into LDS (words is a multiple of 8, the image 16-byte
-// aligned in both places).  The caller synchronises.
-__device__ __forceinline__ void copy_image(const CPlan& c, uint16_t* blob) {
+// aligned in both places) with threads t0, t0 + nt, ...  The caller
+// synchronises.
+__device__ __forceinline__ void copy_image(const CPlan& c, uint16_t* blob, int t0, int nt) {
     const uint4* src = reinterpret_cast<const uint4*>(c.blob);
     uint4* dst = reinterpret_cast<uint4*>(blob);
-    for (int q = lane_id(); q < c.words / 8; q += W) dst[q] = src[q];
+    for (int q = t0; q < c.words / 8; q += nt) dst[q] = src[q];
 }
 
 __device__ __forceinline__ View make_view(double* lds, const uint16_t* blob, const CPlan& c) {
@@ -195,7 +207,7 @@ __device__ __forceinline__ void x_levels(const CPlan& c, const View& v, int o_de
         const uint32_t bd = reinterpret_cast<const uint32_t*>(B + o_desc)[l];
         const int xb = bd & 0xFFFF, nx = (bd >> 16) - xb;
         for (int q = lane_id(); q < nx; q += W) x_task(xt, up, xb + q, v);
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -232,7 +244,7 @@ __device__ __forceinline__ void factor(const CPlan& c, const View& v) {
                 x_task(xt, up, xb + q - nf, v);
             }
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -329,9 +341,9 @@ __device__ __forceinline__ bool newton(const CPlan& c, const View& v, const DevR
     const int tid = lane_id();
     for (int it = 0; it < maxit; it++) {
         eval_mos<NM>(c, v, dr, gmin);
-        __syncthreads();
+        lds_sync();
         assemble<NR>(c, v, a0, qx, false, pinv, flag);
-        __syncthreads();
+        lds_sync();
         factor_solve(c, v);
         nit++;
         double ss = 0.0;
@@ -347,7 +359,7 @@ __device__ __forceinline__ bool newton(const CPlan& c, const View& v, const DevR
             }
         }
         const double nr = sqrt(block_reduce<W>(ss, nullptr, OpSum()) / c.n);
-        __syncthreads();
+        lds_sync();
         if (!(nr == nr)) return false;
         if (nr <= tol) return true;
     }
@@ -370,12 +382,12 @@ __device__ __forceinline__ bool init_point(const CPlan& c, const View& v, const 
     eval_sources(c, v, t0, mode);
     for (int it = 0; it < maxit; it++) {
         eval_mos<NM>(c, v, dr, gmin);
-        __syncthreads();
+        lds_sync();
         const double lm = assemble<NR>(c, v, 0.0, qx0, uic, pinv, flag);
         const double fn = block_reduce<W>(lm, nullptr, OpMax());
         if (fn <= dc_abstol) return true;
         if (!(fn == fn) || __builtin_isinf(fn)) return false;
-        __syncthreads();
+        lds_sync();
         factor_solve(c, v);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -383,7 +395,7 @@ __device__ __forceinline__ bool init_point(const CPlan& c, const View& v, const 
             if (k < c.n) v.u[k] += v.x[k];
         }
         nit++;
-        __syncthreads();
+        lds_sync();
     }
     return false;
 }
@@ -432,7 +444,7 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
             v.u[k] = pinv[r];
         }
     }
-    __syncthreads();
+    lds_sync();
 
     long long steps = 0, rej = 0, nit = 0;
     int rc = init_point<NR, NM>(c, v, dr, gmin, uic, pinv, flag, o.t_start, o.dc_abstol, nit)
@@ -440,7 +452,7 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
 
     double t = o.t_start;
     if (rc == CS_OK) {
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int k = tid + W * r;
@@ -469,7 +481,7 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
                 kout++;
             }
         }
-        __syncthreads();
+        lds_sync();
         while (rc == CS_OK && t < o.t_stop) {
             if (steps + rej >= o.max_steps) { rc = CS_E_MAXSTEPS; break; }
             double lb = INFINITY;
@@ -502,7 +514,7 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
                 }
             }
             eval_sources(c, v, tn, CS_MODE_TRAN);
-            __syncthreads();
+            lds_sync();
             if (!newton<NR, NM>(c, v, dr, gmin, a0, qx, pinv, flag, o.newton_maxit, o.newton_tol,
                                 o.reltol, o.abstol, nit)) {
                 rej++;
@@ -593,7 +605,7 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
         stp->retcode = rc;
         stp->pad = 0;
     }
-    __syncthreads();
+    lds_sync();
 }
 
 }  // namespace lds

@@ -1,9 +1,15 @@
-// tran_lds.hip — the LDS-resident batched transient kernel (k_tran_lds).
+// tran_lds.hip — the LDS-resident batched transient kernels.
 //
-// A persistent grid of one-wavefront workgroups walks a work queue of
-// instances (one size class per launch, largest first).  A workgroup copies
-// its instance's plan image into LDS — skipped when the previous instance
-// used the same circuit — and runs the whole transient there (lds_core.h).
+// k_tran_pack (the batched transient): one launch for the whole batch.  A
+// persistent grid of workgroups of kPackWaves wavefronts walks a queue of
+// chunks; a chunk is up to kPackWaves instances of one circuit.  The
+// workgroup copies the circuit's plan image into LDS once (skipped when the
+// previous chunk used the same circuit) and each wavefront runs one
+// instance's whole transient (lds_core.h) in its own LDS region, so the
+// image is shared and every instance of the batch can be resident at once.
+//
+// k_tran_lds: one wavefront per workgroup and instance (used with size
+// classes; the adjoint's forward pass has the same shape).
 #include "lds_core.h"
 
 namespace cedar {
@@ -26,9 +32,9 @@ __global__ __launch_bounds__(W) void k_tran_lds(LdsArgs A) {
         const int cid = __builtin_amdgcn_readfirstlane(A.inst_cid[inst]);
         const CPlan c = load_header(A.cplans + cid);
         if (cid != cur) {
-            copy_image(c, blob);
+            copy_image(c, blob, lane_id(), W);
             cur = cid;
-            __syncthreads();
+            lds_sync();
         }
         const View v = make_view(lds_mem, blob, c);
         NoRec rec;
@@ -39,7 +45,71 @@ __global__ __launch_bounds__(W) void k_tran_lds(LdsArgs A) {
     }
 }
 
+// Four wavefronts per SIMD (128 VGPRs): two workgroups of eight per CU, so
+// that every instance of a 4096-instance batch is resident at once; what the
+// register limit spills is step-control state outside the level loops.
+template <int NR>
+__global__ __launch_bounds__(W* kPackWaves) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_tran_pack(LdsArgs A) {
+    extern __shared__ double lds_mem[];
+    __shared__ int s_chunk;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint16_t* blob = reinterpret_cast<uint16_t*>(lds_mem);
+    int cur = -1;
+    for (;;) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int ch = __builtin_amdgcn_readfirstlane(s_chunk);
+        __syncthreads();
+        if (ch >= A.n_chunks) break;
+        const int4 cd = A.chunks[ch];
+        const int cid = __builtin_amdgcn_readfirstlane(cd.x);
+        const int first = __builtin_amdgcn_readfirstlane(cd.y);
+        const int count = __builtin_amdgcn_readfirstlane(cd.z);
+        const CPlan c = load_header(A.cplans + cid);
+        if (cid != cur) {
+            copy_image(c, blob, (int)threadIdx.x, W * kPackWaves);
+            cur = cid;
+        }
+        __syncthreads();
+        if (wave < count) {
+            const int inst = __builtin_amdgcn_readfirstlane(A.order[first + wave]);
+            // this wavefront's doubles after the image (words is a multiple of 8)
+            double* region = lds_mem + c.words / 4 + (size_t)wave * ((c.dbl + 1) & ~1);
+            const View v = make_view(region, blob, c);
+            NoRec rec;
+            run_lds<NR>(rec, c, A.plans + cid, A.params + A.p_off[inst],
+                        A.u0 ? A.u0 + A.u_off[inst] : nullptr,
+                        A.wave ? A.wave + A.w_off[inst] : nullptr, A.u_final + A.u_off[inst],
+                        A.stats + inst, A.o, v);
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+int pack_max_blocks(int nr, size_t lds) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (nr) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tran_pack<1>, W * kPackWaves, lds); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tran_pack<2>, W * kPackWaves, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tran_pack<4>, W * kPackWaves, lds); break;
+        default: break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_tran_pack(int nr, const LdsArgs& a, int nblocks, size_t lds, hipStream_t s) {
+    switch (nr) {
+        case 1: hipLaunchKernelGGL(k_tran_pack<1>, dim3(nblocks), dim3(W * kPackWaves), lds, s, a); break;
+        case 2: hipLaunchKernelGGL(k_tran_pack<2>, dim3(nblocks), dim3(W * kPackWaves), lds, s, a); break;
+        case 4: hipLaunchKernelGGL(k_tran_pack<4>, dim3(nblocks), dim3(W * kPackWaves), lds, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 size_t lds_kernel_bytes(int32_t dbl_max, int32_t words_max) {
     return 8 * (size_t)dbl_max + 2 * (size_t)words_max;

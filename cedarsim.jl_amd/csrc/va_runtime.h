@@ -1,0 +1,301 @@
+// va_runtime.h — scalar runtime of the generated Verilog-A model code
+// (cedarsim.jl_amd/vagen): a forward-mode dual number carrying the partial
+// derivatives with respect to the device's node voltages, and the Verilog-A
+// math functions for double and dual arguments.
+//
+// Domain rules follow src/va_env.jl: ln, sqrt and pow return NaN outside
+// their real domains (NaNMath), real-to-integer conversion rounds half away
+// from zero (VA-LRM 4.2.1.1), comparisons look at values only.
+#pragma once
+#include <cmath>
+#include <cstddef>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CEDAR_VA_HD __host__ __device__
+#else
+#define CEDAR_VA_HD
+#endif
+
+template <int N>
+struct VaDual {
+    double v;
+    double d[N];
+    CEDAR_VA_HD VaDual() : v(0.0) {
+        for (int i = 0; i < N; i++) d[i] = 0.0;
+    }
+    CEDAR_VA_HD VaDual(double x) : v(x) {
+        for (int i = 0; i < N; i++) d[i] = 0.0;
+    }
+    CEDAR_VA_HD static VaDual var(double x, int k) {
+        VaDual r(x);
+        r.d[k] = 1.0;
+        return r;
+    }
+    CEDAR_VA_HD VaDual& operator+=(const VaDual& b) {
+        v += b.v;
+        for (int i = 0; i < N; i++) d[i] += b.d[i];
+        return *this;
+    }
+    CEDAR_VA_HD VaDual& operator-=(const VaDual& b) {
+        v -= b.v;
+        for (int i = 0; i < N; i++) d[i] -= b.d[i];
+        return *this;
+    }
+};
+
+// ---- value access ------------------------------------------------------------
+CEDAR_VA_HD inline double va_val(double x) { return x; }
+CEDAR_VA_HD inline double va_val(int x) { return (double)x; }
+template <int N>
+CEDAR_VA_HD inline double va_val(const VaDual<N>& x) { return x.v; }
+
+CEDAR_VA_HD inline bool va_truth(int x) { return x != 0; }
+CEDAR_VA_HD inline bool va_truth(bool x) { return x; }
+CEDAR_VA_HD inline bool va_truth(double x) { return x != 0.0; }
+template <int N>
+CEDAR_VA_HD inline bool va_truth(const VaDual<N>& x) { return x.v != 0.0; }
+
+template <class X>
+CEDAR_VA_HD inline int va_to_int(const X& x) {
+    const double v = va_val(x);
+    return (int)(v < 0.0 ? -std::floor(-v + 0.5) : std::floor(v + 0.5));
+}
+CEDAR_VA_HD inline int va_to_int(int x) { return x; }
+
+// ---- arithmetic -----------------------------------------------------------------
+#define VA_DUAL_OP(OP, VAL, DER)                                                   \
+    template <int N>                                                              \
+    CEDAR_VA_HD inline VaDual<N> operator OP(const VaDual<N>& a, const VaDual<N>& b) { \
+        VaDual<N> r;                                                              \
+        r.v = VAL;                                                                \
+        for (int i = 0; i < N; i++) r.d[i] = DER;                                 \
+        return r;                                                                 \
+    }
+VA_DUAL_OP(+, a.v + b.v, a.d[i] + b.d[i])
+VA_DUAL_OP(-, a.v - b.v, a.d[i] - b.d[i])
+VA_DUAL_OP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+#undef VA_DUAL_OP
+
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator/(const VaDual<N>& a, const VaDual<N>& b) {
+    VaDual<N> r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+    for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator-(const VaDual<N>& a) {
+    VaDual<N> r;
+    r.v = -a.v;
+    for (int i = 0; i < N; i++) r.d[i] = -a.d[i];
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator+(const VaDual<N>& a) { return a; }
+
+// mixed dual / scalar
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator+(const VaDual<N>& a, double b) {
+    VaDual<N> r = a;
+    r.v += b;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator+(double a, const VaDual<N>& b) { return b + a; }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator-(const VaDual<N>& a, double b) {
+    VaDual<N> r = a;
+    r.v -= b;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator-(double a, const VaDual<N>& b) {
+    VaDual<N> r;
+    r.v = a - b.v;
+    for (int i = 0; i < N; i++) r.d[i] = -b.d[i];
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator*(const VaDual<N>& a, double b) {
+    VaDual<N> r;
+    r.v = a.v * b;
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator*(double a, const VaDual<N>& b) { return b * a; }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator/(const VaDual<N>& a, double b) {
+    VaDual<N> r;
+    r.v = a.v / b;
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] / b;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> operator/(double a, const VaDual<N>& b) {
+    VaDual<N> r;
+    r.v = a / b.v;
+    const double k = -r.v / b.v;
+    for (int i = 0; i < N; i++) r.d[i] = k * b.d[i];
+    return r;
+}
+// int operands promote through double
+#define VA_INT_MIX(OP)                                                                        \
+    template <int N>                                                                         \
+    CEDAR_VA_HD inline VaDual<N> operator OP(const VaDual<N>& a, int b) { return a OP (double)b; } \
+    template <int N>                                                                         \
+    CEDAR_VA_HD inline VaDual<N> operator OP(int a, const VaDual<N>& b) { return (double)a OP b; }
+VA_INT_MIX(+)
+VA_INT_MIX(-)
+VA_INT_MIX(*)
+VA_INT_MIX(/)
+#undef VA_INT_MIX
+
+// comparisons look at values
+#define VA_CMP(OP)                                                                              \
+    template <int N>                                                                           \
+    CEDAR_VA_HD inline bool operator OP(const VaDual<N>& a, const VaDual<N>& b) { return a.v OP b.v; } \
+    template <int N>                                                                           \
+    CEDAR_VA_HD inline bool operator OP(const VaDual<N>& a, double b) { return a.v OP b; }     \
+    template <int N>                                                                           \
+    CEDAR_VA_HD inline bool operator OP(double a, const VaDual<N>& b) { return a OP b.v; }     \
+    template <int N>                                                                           \
+    CEDAR_VA_HD inline bool operator OP(const VaDual<N>& a, int b) { return a.v OP (double)b; } \
+    template <int N>                                                                           \
+    CEDAR_VA_HD inline bool operator OP(int a, const VaDual<N>& b) { return (double)a OP b.v; }
+VA_CMP(<)
+VA_CMP(<=)
+VA_CMP(>)
+VA_CMP(>=)
+VA_CMP(==)
+VA_CMP(!=)
+#undef VA_CMP
+
+// ---- functions: f(x) with derivative f'(x) --------------------------------------
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_chain(const VaDual<N>& x, double fx, double dfx) {
+    VaDual<N> r;
+    r.v = fx;
+    for (int i = 0; i < N; i++) r.d[i] = dfx * x.d[i];
+    return r;
+}
+
+CEDAR_VA_HD inline double va_exp(double x) { return std::exp(x); }
+CEDAR_VA_HD inline double va_ln(double x) { return x < 0.0 ? NAN : std::log(x); }
+CEDAR_VA_HD inline double va_log10(double x) { return x < 0.0 ? NAN : std::log10(x); }
+CEDAR_VA_HD inline double va_sqrt(double x) { return x < 0.0 ? NAN : std::sqrt(x); }
+CEDAR_VA_HD inline double va_abs(double x) { return std::fabs(x); }
+CEDAR_VA_HD inline int va_abs(int x) { return x < 0 ? -x : x; }
+CEDAR_VA_HD inline double va_sin(double x) { return std::sin(x); }
+CEDAR_VA_HD inline double va_cos(double x) { return std::cos(x); }
+CEDAR_VA_HD inline double va_tan(double x) { return std::tan(x); }
+CEDAR_VA_HD inline double va_atan(double x) { return std::atan(x); }
+CEDAR_VA_HD inline double va_asin(double x) { return std::asin(x); }
+CEDAR_VA_HD inline double va_acos(double x) { return std::acos(x); }
+CEDAR_VA_HD inline double va_sinh(double x) { return std::sinh(x); }
+CEDAR_VA_HD inline double va_cosh(double x) { return std::cosh(x); }
+CEDAR_VA_HD inline double va_tanh(double x) { return std::tanh(x); }
+CEDAR_VA_HD inline double va_asinh(double x) { return std::asinh(x); }
+CEDAR_VA_HD inline double va_floor(double x) { return std::floor(x); }
+CEDAR_VA_HD inline double va_ceil(double x) { return std::ceil(x); }
+CEDAR_VA_HD inline double va_limexp(double x) { return std::exp(x); }
+CEDAR_VA_HD inline double va_fmod(double a, double b) { return std::fmod(a, b); }
+CEDAR_VA_HD inline double va_pow(double a, double b) {
+    // NaNMath.pow: NaN for a negative base with a non-integer exponent
+    if (a < 0.0 && std::floor(b) != b) return NAN;
+    return std::pow(a, b);
+}
+CEDAR_VA_HD inline double va_atan2(double a, double b) { return std::atan2(a, b); }
+CEDAR_VA_HD inline double va_hypot(double a, double b) { return std::hypot(a, b); }
+
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_exp(const VaDual<N>& x) {
+    const double e = std::exp(x.v);
+    return va_chain(x, e, e);
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_limexp(const VaDual<N>& x) { return va_exp(x); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_ln(const VaDual<N>& x) { return va_chain(x, va_ln(x.v), 1.0 / x.v); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_log10(const VaDual<N>& x) {
+    return va_chain(x, va_log10(x.v), 1.0 / (x.v * 2.302585092994045684));
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_sqrt(const VaDual<N>& x) {
+    const double s = va_sqrt(x.v);
+    return va_chain(x, s, 0.5 / s);
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_abs(const VaDual<N>& x) { return x.v < 0.0 ? -x : x; }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_sin(const VaDual<N>& x) { return va_chain(x, std::sin(x.v), std::cos(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_cos(const VaDual<N>& x) { return va_chain(x, std::cos(x.v), -std::sin(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_tan(const VaDual<N>& x) {
+    const double t = std::tan(x.v);
+    return va_chain(x, t, 1.0 + t * t);
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_atan(const VaDual<N>& x) { return va_chain(x, std::atan(x.v), 1.0 / (1.0 + x.v * x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_asin(const VaDual<N>& x) {
+    return va_chain(x, std::asin(x.v), 1.0 / std::sqrt(1.0 - x.v * x.v));
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_acos(const VaDual<N>& x) {
+    return va_chain(x, std::acos(x.v), -1.0 / std::sqrt(1.0 - x.v * x.v));
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_sinh(const VaDual<N>& x) { return va_chain(x, std::sinh(x.v), std::cosh(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_cosh(const VaDual<N>& x) { return va_chain(x, std::cosh(x.v), std::sinh(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_tanh(const VaDual<N>& x) {
+    const double t = std::tanh(x.v);
+    return va_chain(x, t, 1.0 - t * t);
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_asinh(const VaDual<N>& x) {
+    return va_chain(x, std::asinh(x.v), 1.0 / std::sqrt(x.v * x.v + 1.0));
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_floor(const VaDual<N>& x) { return VaDual<N>(std::floor(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_ceil(const VaDual<N>& x) { return VaDual<N>(std::ceil(x.v)); }
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_pow(const VaDual<N>& a, const VaDual<N>& b) {
+    // d(a^b) = b a^(b-1) da + a^b ln(a) db; the db term is dropped when b
+    // carries no derivative (a strong zero, as in src/va_env.jl)
+    const double y = va_pow(a.v, b.v);
+    bool db = false;
+    for (int i = 0; i < N; i++) db = db || (b.d[i] != 0.0);
+    const double dx = b.v == 0.0 ? 0.0 : b.v * va_pow(a.v, b.v - 1.0);
+    const double dp = db ? y * va_ln(a.v) : 0.0;
+    VaDual<N> r;
+    r.v = y;
+    for (int i = 0; i < N; i++) r.d[i] = dx * a.d[i] + (db ? dp * b.d[i] : 0.0);
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_atan2(const VaDual<N>& a, const VaDual<N>& b) {
+    const double den = a.v * a.v + b.v * b.v;
+    VaDual<N> r;
+    r.v = std::atan2(a.v, b.v);
+    for (int i = 0; i < N; i++) r.d[i] = (b.v * a.d[i] - a.v * b.d[i]) / den;
+    return r;
+}
+template <int N>
+CEDAR_VA_HD inline VaDual<N> va_hypot(const VaDual<N>& a, const VaDual<N>& b) {
+    return va_sqrt(a * a + b * b);
+}
+
+// min / max follow the selected operand (derivative included); ties pick the
+// first operand for min and the second for max, as Julia's Base.min / max
+template <class X>
+CEDAR_VA_HD inline X va_min(const X& a, const X& b) { return va_val(b) < va_val(a) ? b : a; }
+template <class X>
+CEDAR_VA_HD inline X va_max(const X& a, const X& b) { return va_val(a) > va_val(b) ? a : b; }

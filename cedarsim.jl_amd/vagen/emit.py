@@ -1,0 +1,552 @@
+"""C++ emitter: one header per model with
+  * a parameter struct (model card, shared by instances) and an instance
+    struct (per-instance parameters such as L and NFIN), each with the
+    model's default expressions evaluated in declaration order for the
+    parameters that are not given;
+  * eval<T>(card, inst, temp, V[], I[], Q[]): every branch contribution with
+    the node voltages V as scalars of type T, accumulated as resistive
+    currents I and charges Q (the arguments of ddt) leaving each node.
+Variables whose values depend on a node voltage get type T; the others stay
+double (a fixed point over all assignments), so that a dual-number T only
+pays for the bias-dependent part.  Semantics follow src/va_env.jl: ln, sqrt
+and pow return NaN outside their domains, real-to-integer conversion rounds
+half away from zero, $temperature is in kelvin, noise sources are zero."""
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Set, Tuple
+
+from .parse import (Assign, Bin, Block, Call, Case, Contrib, For, Function, If, Module, Num, Probe,
+                    Str, Task, Tern, Un, Var)
+
+MATH1 = {"exp": "va_exp", "ln": "va_ln", "log": "va_log10", "sqrt": "va_sqrt", "abs": "va_abs",
+         "sin": "va_sin", "cos": "va_cos", "tan": "va_tan", "atan": "va_atan", "sinh": "va_sinh",
+         "cosh": "va_cosh", "tanh": "va_tanh", "floor": "va_floor", "ceil": "va_ceil",
+         "limexp": "va_limexp", "asinh": "va_asinh", "acos": "va_acos", "asin": "va_asin",
+         "$ln": "va_ln", "$log10": "va_log10", "$exp": "va_exp", "$sqrt": "va_sqrt",
+         "$abs": "va_abs"}
+MATH2 = {"pow": "va_pow", "atan2": "va_atan2", "hypot": "va_hypot", "$pow": "va_pow"}
+CXX_RESERVED = {"int", "double", "float", "char", "long", "short", "auto", "default", "case",
+                "switch", "new", "delete", "class", "struct", "template", "this", "operator",
+                "union", "enum", "register", "signed", "unsigned", "volatile", "const", "static",
+                "extern", "inline", "friend", "public", "private", "protected", "virtual",
+                "namespace", "using", "try", "catch", "throw", "typename", "typedef", "goto",
+                "void", "bool", "true", "false", "sizeof", "break", "continue", "return", "do",
+                "while", "for", "if", "else", "P", "Ip", "V", "I", "Q", "T", "temp"}
+
+
+class EmitError(Exception):
+    pass
+
+
+def cname(n: str) -> str:
+    return "v_" + n if n in CXX_RESERVED or n.startswith("_") else n
+
+
+def num_lit(v) -> str:
+    if isinstance(v, int):
+        return str(v)
+    s = repr(float(v))
+    if "inf" in s or "nan" in s:
+        raise EmitError("non-finite literal")
+    return s if ("." in s or "e" in s) else s + ".0"
+
+
+class Emitter:
+    def __init__(self, mod: Module, inst_params: List[str], prefix: str):
+        self.m = mod
+        self.prefix = prefix
+        self.params = {p.name: p for p in mod.params}
+        # a parameter whose default depends on an instance parameter is
+        # itself per instance
+        inst_set = set(inst_params)
+        changed = True
+        while changed:
+            changed = False
+            for p in mod.params:
+                if p.name not in inst_set and (refs(p.default) & inst_set):
+                    inst_set.add(p.name)
+                    changed = True
+        self.inst = [p for p in mod.params if p.name in inst_set]
+        self.card = [p for p in mod.params if p.name not in inst_set]
+        self.inst_set = {p.name for p in self.inst}
+        self.node_idx = {n: i for i, n in enumerate(mod.nodes)}
+        self.dual: Set[str] = set()
+        self.fn_scope: Optional[Function] = None
+        self.fn_dual_ret: Dict[str, bool] = {}
+
+    # ---- typing ---------------------------------------------------------------
+    def var_type(self, name: str) -> str:
+        if self.fn_scope is not None:
+            f = self.fn_scope
+            if name == f.name:
+                return "dual" if self.fn_local_dual.get(name) else ("int" if f.rtype == "integer" else "real")
+            if name in f.locals or name in f.inputs:
+                if self.fn_local_dual.get(name):
+                    return "dual"
+                return "int" if f.locals.get(name) == "integer" else "real"
+        if name in self.params:
+            return "int" if self.params[name].type == "integer" else "real"
+        if name in self.m.vars:
+            if self.m.vars[name] == "integer":
+                return "int"
+            return "dual" if name in self.dual else "real"
+        raise EmitError(f"unknown identifier {name}")
+
+    def ty(self, e) -> str:
+        if isinstance(e, Num):
+            return "int" if isinstance(e.v, int) else "real"
+        if isinstance(e, Str):
+            return "str"
+        if isinstance(e, Var):
+            if e.name.startswith("$"):
+                return self.ty(Call(e.name, []))
+            return self.var_type(e.name)
+        if isinstance(e, Probe):
+            if e.kind != "V":
+                raise EmitError("current probes are not supported")
+            return "dual"
+        if isinstance(e, Un):
+            t = self.ty(e.a)
+            return "int" if e.op in ("!", "~") else t
+        if isinstance(e, Bin):
+            if e.op in ("==", "!=", "<", "<=", ">", ">=", "&&", "||"):
+                return "int"
+            if e.op in ("&", "|", "^", "<<", ">>"):
+                return "int"
+            a, b = self.ty(e.a), self.ty(e.b)
+            if e.op == "**":
+                return "dual" if "dual" in (a, b) else "real"
+            return promote(a, b)
+        if isinstance(e, Tern):
+            return promote(self.ty(e.a), self.ty(e.b))
+        if isinstance(e, Call):
+            n = e.name
+            ats = [self.ty(a) for a in e.args]
+            if n in MATH1 or n in MATH2:
+                return "dual" if "dual" in ats else "real"
+            if n in ("min", "max", "$min", "$max"):
+                return promote(*ats)
+            if n in ("$temperature", "$vt"):
+                return "real"
+            if n in ("$param_given", "param_given"):
+                return "int"
+            if n in ("white_noise", "flicker_noise", "noise_table"):
+                return "real"
+            if n in ("ddt", "ddx"):
+                return "dual"
+            if n in self.m.functions:
+                f = self.m.functions[n]
+                if f.rtype == "integer":
+                    return "int"
+                return "dual" if "dual" in ats else "real"
+            raise EmitError(f"unsupported function {n}")
+        raise EmitError(f"cannot type {e}")
+
+    def infer(self):
+        """Fixed point: a real variable is dual when any assignment to it has
+        a dual right-hand side."""
+        changed = True
+        while changed:
+            changed = False
+            for name, expr in self.all_assigns():
+                if self.m.vars.get(name) == "real" and name not in self.dual and self.ty(expr) == "dual":
+                    self.dual.add(name)
+                    changed = True
+
+    def all_assigns(self):
+        out = []
+
+        def walk(s):
+            if isinstance(s, Block):
+                for x in s.stmts:
+                    walk(x)
+            elif isinstance(s, If):
+                walk(s.t)
+                if s.e is not None:
+                    walk(s.e)
+            elif isinstance(s, Case):
+                for _, st in s.items:
+                    walk(st)
+            elif isinstance(s, For):
+                walk(s.init)
+                walk(s.step)
+                walk(s.body)
+            elif isinstance(s, Assign):
+                out.append((s.name, s.expr))
+        for s in self.m.analog:
+            walk(s)
+        return out
+
+    # ---- expressions ------------------------------------------------------------
+    def cxx_type(self, t: str) -> str:
+        return {"int": "int", "real": "double", "dual": "T"}[t]
+
+    def cast(self, e, to: str) -> str:
+        s = self.ex(e)
+        t = self.ty(e)
+        if t == to:
+            return s
+        if to == "dual":
+            return f"T({s})"
+        if to == "real":
+            return f"double({s})"
+        if to == "int":
+            return f"va_to_int({s})" if t != "int" else s
+        raise EmitError("bad cast")
+
+    def node(self, n: str) -> str:
+        if n not in self.node_idx:
+            raise EmitError(f"unknown node {n}")
+        return f"V[{self.node_idx[n]}]"
+
+    def ex(self, e) -> str:
+        if isinstance(e, Num):
+            return num_lit(e.v)
+        if isinstance(e, Str):
+            return '""'
+        if isinstance(e, Var):
+            n = e.name
+            if n.startswith("$"):
+                return self.ex(Call(n, []))
+            if self.fn_scope is not None and (n in self.fn_scope.locals or n in self.fn_scope.inputs
+                                              or n == self.fn_scope.name):
+                return cname(n)
+            if n in self.params:
+                return ("Ip." if n in self.inst_set else "P.") + cname(n)
+            return cname(n)
+        if isinstance(e, Probe):
+            if e.b is None:
+                return self.node(e.a)
+            return f"({self.node(e.a)} - {self.node(e.b)})"
+        if isinstance(e, Un):
+            if e.op == "!":
+                return f"(!va_truth({self.ex(e.a)}))"
+            if e.op == "~":
+                return f"(~{self.cast(e.a, 'int')})"
+            return f"({e.op}{self.ex(e.a)})"
+        if isinstance(e, Bin):
+            op = e.op
+            if op == "**":
+                t = self.ty(e)
+                return f"va_pow({self.cast(e.a, t)}, {self.cast(e.b, t)})"
+            if op in ("&&", "||"):
+                return f"(va_truth({self.ex(e.a)}) {op} va_truth({self.ex(e.b)}))"
+            if op in ("&", "|", "^", "<<", ">>"):
+                return f"({self.cast(e.a, 'int')} {op} {self.cast(e.b, 'int')})"
+            if op == "%":
+                ta, tb = self.ty(e.a), self.ty(e.b)
+                if ta == "int" and tb == "int":
+                    return f"({self.ex(e.a)} % {self.ex(e.b)})"
+                return f"va_fmod({self.cast(e.a, 'real')}, {self.cast(e.b, 'real')})"
+            return f"({self.ex(e.a)} {op} {self.ex(e.b)})"
+        if isinstance(e, Tern):
+            t = self.ty(e)
+            return f"(va_truth({self.ex(e.c)}) ? {self.cast(e.a, t)} : {self.cast(e.b, t)})"
+        if isinstance(e, Call):
+            n = e.name
+            if n in MATH1:
+                return f"{MATH1[n]}({self.ex(e.args[0])})"
+            if n in MATH2:
+                t = "dual" if self.ty(e) == "dual" else "real"
+                return f"{MATH2[n]}({self.cast(e.args[0], t)}, {self.cast(e.args[1], t)})"
+            if n in ("min", "max", "$min", "$max"):
+                t = self.ty(e)
+                return (f"va_{n.strip('$')}<{self.cxx_type(t)}>"
+                        f"({self.cast(e.args[0], t)}, {self.cast(e.args[1], t)})")
+            if n == "$temperature":
+                return "temp"
+            if n == "$vt":
+                tt = self.cast(e.args[0], "real") if e.args else "temp"
+                return f"(1.3806503e-23 * {tt} / 1.602176462e-19)"
+            if n in ("$param_given", "param_given"):
+                a = e.args[0]
+                if not isinstance(a, Var) or a.name not in self.params:
+                    raise EmitError("$param_given needs a parameter")
+                pre = "Ip." if a.name in self.inst_set else "P."
+                return f"int({pre}given_{a.name})"
+            if n in ("white_noise", "flicker_noise", "noise_table"):
+                return "0.0"
+            if n in self.m.functions:
+                f = self.m.functions[n]
+                t = self.ty(e)
+                ct = self.cxx_type("dual" if t == "dual" else ("int" if f.rtype == "integer" else "real"))
+                at = "dual" if t == "dual" else "real"
+                args = ", ".join(self.cast(a, at) if self.fn_input_real(f, i) else self.cast(a, "int")
+                                 for i, a in enumerate(e.args))
+                return f"{self.prefix}_fn_{n}<{ct if t == 'dual' else 'double'}>({args})"
+            raise EmitError(f"unsupported function {n}")
+        raise EmitError(f"cannot emit {e}")
+
+    def fn_input_real(self, f: Function, i: int) -> bool:
+        return f.locals.get(f.inputs[i], "real") != "integer"
+
+    # ---- contributions -------------------------------------------------------------
+    def split(self, e) -> Tuple[Optional[object], Optional[object]]:
+        """(resistive part, charge part) of a contribution expression."""
+        if isinstance(e, Call) and e.name == "ddt":
+            return None, e.args[0]
+        if isinstance(e, Bin) and e.op in ("+", "-"):
+            ra, qa = self.split(e.a)
+            rb, qb = self.split(e.b)
+            return combine(ra, rb, e.op), combine(qa, qb, e.op)
+        if isinstance(e, Un) and e.op == "-":
+            r, q = self.split(e.a)
+            return (Un("-", r) if r is not None else None), (Un("-", q) if q is not None else None)
+        if isinstance(e, Bin) and e.op in ("*", "/"):
+            ra, qa = self.split(e.a)
+            if e.op == "*":
+                rb, qb = self.split(e.b)
+                if qa is not None and qb is not None:
+                    raise EmitError("product of ddt terms")
+                if qa is not None:
+                    if rb is None or ra is not None:
+                        raise EmitError("ddt term times a ddt term")
+                    return (Bin("*", ra, e.b) if ra is not None else None), Bin("*", qa, e.b)
+                if qb is not None:
+                    if ra is None or rb is not None:
+                        raise EmitError("ddt term times a ddt term")
+                    return None, Bin("*", e.a, qb)
+                return e, None
+            if qa is not None:
+                if contains_ddt(e.b):
+                    raise EmitError("division by a ddt term")
+                return (Bin("/", ra, e.b) if ra is not None else None), Bin("/", qa, e.b)
+            return e, None
+        if contains_ddt(e):
+            raise EmitError("ddt inside a nonlinear expression")
+        return e, None
+
+    # ---- statements -------------------------------------------------------------------
+    def st(self, s, ind: str, out: List[str]):
+        if isinstance(s, Block):
+            out.append(ind + "{")
+            for x in s.stmts:
+                self.st(x, ind + "    ", out)
+            out.append(ind + "}")
+        elif isinstance(s, If):
+            out.append(f"{ind}if (va_truth({self.ex(s.c)})) {{")
+            self.st(s.t, ind + "    ", out)
+            if s.e is not None:
+                out.append(ind + "} else {")
+                self.st(s.e, ind + "    ", out)
+            out.append(ind + "}")
+        elif isinstance(s, Case):
+            out.append(ind + "{")
+            out.append(f"{ind}    const auto va_sel = {self.ex(s.expr)};")
+            first = True
+            dflt = None
+            for labels, stmt in s.items:
+                if labels is None:
+                    dflt = stmt
+                    continue
+                cond = " || ".join(f"va_sel == {self.ex(l)}" for l in labels)
+                out.append(f"{ind}    {'if' if first else '} else if'} ({cond}) {{")
+                self.st(stmt, ind + "        ", out)
+                first = False
+            if dflt is not None:
+                if first:
+                    self.st(dflt, ind + "    ", out)
+                else:
+                    out.append(ind + "    } else {")
+                    self.st(dflt, ind + "        ", out)
+            if not first:
+                out.append(ind + "    }")
+            out.append(ind + "}")
+        elif isinstance(s, For):
+            out.append(ind + "{")
+            self.st(s.init, ind + "    ", out)
+            out.append(f"{ind}    while (va_truth({self.ex(s.cond)})) {{")
+            self.st(s.body, ind + "        ", out)
+            self.st(s.step, ind + "        ", out)
+            out.append(ind + "    }")
+            out.append(ind + "}")
+        elif isinstance(s, Assign):
+            t = self.var_type(s.name)
+            if t == "int":
+                rhs = self.cast(s.expr, "int")
+            elif t == "real":
+                if self.ty(s.expr) == "dual":
+                    raise EmitError(f"dual value assigned to real {s.name}")
+                rhs = self.cast(s.expr, "real")
+            else:
+                rhs = self.cast(s.expr, "dual")
+            out.append(f"{ind}{cname(s.name)} = {rhs};")
+        elif isinstance(s, Contrib):
+            if s.kind != "I":
+                raise EmitError("voltage contributions are not supported")
+            r, q = self.split(s.expr)
+            a = self.node_idx[s.a]
+            b = self.node_idx[s.b] if s.b is not None else None
+            for part, arr in ((r, "I"), (q, "Q")):
+                if part is None:
+                    continue
+                val = self.cast(part, "dual")
+                out.append(f"{ind}{{ const T va_c = {val}; {arr}[{a}] += va_c;"
+                           + (f" {arr}[{b}] -= va_c;" if b is not None else "") + " }")
+        elif isinstance(s, Task):
+            return
+        else:
+            raise EmitError(f"cannot emit statement {s}")
+
+    # ---- functions ----------------------------------------------------------------------
+    def function(self, f: Function) -> str:
+        # a function is emitted once, templated on the scalar type of its
+        # real inputs; locals are of that type (they may depend on inputs)
+        self.fn_scope = f
+        self.fn_local_dual = {n: (f.locals.get(n, "real") != "integer") for n in f.locals}
+        self.fn_local_dual.update({n: True for n in f.inputs if f.locals.get(n, "real") != "integer"})
+        self.fn_local_dual[f.name] = f.rtype != "integer"
+        lines = []
+        args = ", ".join((("S " if f.locals.get(n, "real") != "integer" else "int ") + cname(n))
+                         for n in f.inputs)
+        rt = "S" if f.rtype != "integer" else "int"
+        lines.append(f"template <class S> CEDAR_VA_HD inline {rt} {self.prefix}_fn_{f.name}({args}) {{")
+        lines.append("    using T = S;")
+        for n, t in f.locals.items():
+            if n in f.inputs:
+                continue
+            lines.append(f"    {'S' if t != 'integer' else 'int'} {cname(n)} = 0;")
+        lines.append(f"    {rt} {cname(f.name)} = 0;")
+        self.st(f.body, "    ", lines)
+        lines.append(f"    return {cname(f.name)};")
+        lines.append("}")
+        self.fn_scope = None
+        return "\n".join(lines)
+
+    def ty_in_fn(self):
+        pass
+
+    # ---- parameter defaults -------------------------------------------------------------
+    def param_default(self, p, target: str) -> str:
+        e = p.default
+        t = "int" if p.type == "integer" else "real"
+        return self.cast(e, t)
+
+    # ---- module -------------------------------------------------------------------------
+    def emit(self) -> str:
+        self.infer()
+        pf = self.prefix
+        o = []
+        o.append(f"// {pf}: generated by vagen from Verilog-A module '{self.m.name}'.")
+        o.append("// Do not edit; regenerate from the model file.")
+        o.append("#pragma once")
+        o.append('#include "va_runtime.h"')
+        o.append("")
+        o.append(f"namespace va_{pf} {{")
+        o.append(f"constexpr int n_nodes = {len(self.m.nodes)};")
+        o.append(f"constexpr int n_ports = {len(self.m.ports)};")
+        o.append("static const char* const node_names[] = {" +
+                 ", ".join(f'"{n}"' for n in self.m.nodes) + "};")
+        for kind, plist in (("Card", self.card), ("Inst", self.inst)):
+            o.append(f"struct {kind} {{")
+            for p in plist:
+                o.append(f"    {'int' if p.type == 'integer' else 'double'} {cname(p.name)};")
+            for p in plist:
+                o.append(f"    unsigned char given_{p.name};")
+            o.append("};")
+        # parameter tables for setting by name from the host
+        for kind, plist in (("Card", self.card), ("Inst", self.inst)):
+            o.append(f"static const struct {{ const char* name; int is_int; int off; int given_off; }} "
+                     f"{kind.lower()}_table[] = {{")
+            for p in plist:
+                o.append(f'    {{"{p.name}", {1 if p.type == "integer" else 0}, '
+                         f'(int)offsetof({kind}, {cname(p.name)}), (int)offsetof({kind}, given_{p.name})}},')
+            o.append("};")
+            o.append(f"constexpr int n_{kind.lower()} = {len(plist)};")
+        # defaults: card first (card defaults may only reference card params),
+        # then instance params (may reference both)
+        o.append("inline void card_defaults(Card& P) {")
+        o.append("    Inst Ip{};  (void)Ip;")
+        for p in self.card:
+            self.check_default_refs(p, allow_inst=False)
+            o.append(f"    if (!P.given_{p.name}) P.{cname(p.name)} = {self.param_default(p, 'card')};")
+        o.append("}")
+        o.append("CEDAR_VA_HD inline void inst_defaults(const Card& P, Inst& Ip) {")
+        for p in self.inst:
+            self.check_default_refs(p, allow_inst=True)
+            o.append(f"    if (!Ip.given_{p.name}) Ip.{cname(p.name)} = {self.param_default(p, 'inst')};")
+        o.append("}")
+        o.append("")
+        for f in self.m.functions.values():
+            o.append(self.function(f))
+            o.append("")
+        # eval
+        o.append("// I[k], Q[k]: resistive current and charge leaving node k (KCL order of node_names)")
+        o.append("template <class T>")
+        o.append("CEDAR_VA_HD inline void eval(const Card& P, const Inst& Ip, double temp, const T* V, T* I, T* Q) {")
+        o.append("    for (int k = 0; k < n_nodes; k++) { I[k] = T(0.0); Q[k] = T(0.0); }")
+        for n, t in self.m.vars.items():
+            ct = "int" if t == "integer" else ("T" if n in self.dual else "double")
+            o.append(f"    {ct} {cname(n)} = 0;")
+        body: List[str] = []
+        for s in self.m.analog:
+            self.st(s, "    ", body)
+        o += body
+        o.append("}")
+        o.append(f"}}  // namespace va_{pf}")
+        return "\n".join(o) + "\n"
+
+    def check_default_refs(self, p, allow_inst: bool):
+        def walk(e):
+            if isinstance(e, Var):
+                if e.name not in self.params:
+                    raise EmitError(f"default of {p.name} references {e.name}")
+                if not allow_inst and e.name in self.inst_set:
+                    raise EmitError(f"card parameter {p.name} defaults to instance parameter {e.name}")
+            for ch in getattr(e, "__dict__", {}).values():
+                if isinstance(ch, (list, tuple)):
+                    for c in ch:
+                        walk(c)
+                elif hasattr(ch, "__dict__"):
+                    walk(ch)
+        walk(p.default)
+
+
+def refs(e) -> Set[str]:
+    out: Set[str] = set()
+
+    def walk(x):
+        if isinstance(x, Var):
+            out.add(x.name)
+        for ch in getattr(x, "__dict__", {}).values():
+            if isinstance(ch, list):
+                for c in ch:
+                    walk(c)
+            elif hasattr(ch, "__dict__"):
+                walk(ch)
+    walk(e)
+    return out
+
+
+def promote(*ts: str) -> str:
+    if "dual" in ts:
+        return "dual"
+    if "real" in ts:
+        return "real"
+    return "int"
+
+
+def combine(a, b, op):
+    if a is None and b is None:
+        return None
+    if a is None:
+        return b if op == "+" else Un("-", b)
+    if b is None:
+        return a
+    return Bin(op, a, b)
+
+
+def contains_ddt(e) -> bool:
+    if isinstance(e, Call) and e.name == "ddt":
+        return True
+    for ch in getattr(e, "__dict__", {}).values():
+        if isinstance(ch, list):
+            if any(contains_ddt(c) for c in ch):
+                return True
+        elif hasattr(ch, "__dict__") and contains_ddt(ch):
+            return True
+    return False
+
+
+def emit_cpp(mod: Module, inst_params: List[str], prefix: str) -> str:
+    return Emitter(mod, inst_params, prefix).emit()

@@ -1,0 +1,563 @@
+"""Lexer and recursive-descent parser for the Verilog-A subset that compact
+models use: one module with ports and internal nodes, real/integer
+parameters (defaults may reference earlier parameters; ranges are kept for
+documentation only), variables, analog functions, and one analog block of
+begin/end (optionally named, with local declarations), if/else, case, for,
+assignments, branch contributions I(a,b) <+ ... / V(a,b) <+ ..., system
+tasks ($strobe and friends, ignored), and event controls (@(...) stmt)."""
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+SI = {"T": 1e12, "G": 1e9, "M": 1e6, "K": 1e3, "k": 1e3, "m": 1e-3, "u": 1e-6, "n": 1e-9,
+      "p": 1e-12, "f": 1e-15, "a": 1e-18}
+
+
+class ParseError(Exception):
+    pass
+
+
+# ---- tokens ------------------------------------------------------------------
+@dataclass
+class Tok:
+    kind: str   # 'id', 'sys', 'num', 'str', 'op', 'eof'
+    text: str
+    val: object = None
+    line: int = 0
+
+
+_num = re.compile(r"(\d+\.\d*|\.\d+|\d+)([eE][+-]?\d+)?([TGMKkmunpfa](?![A-Za-z0-9_]))?")
+_ident = re.compile(r"[A-Za-z_][A-Za-z0-9_]*")
+_ops = ["<+", "**", "==", "!=", "<=", ">=", "&&", "||", "<<", ">>", "(*", "*)",
+        "+", "-", "*", "/", "%", "<", ">", "!", "~", "&", "|", "^", "?", ":", ",", ";",
+        "(", ")", "[", "]", "{", "}", "=", "@", "#", "."]
+
+
+def lex(text: str) -> List[Tok]:
+    toks, i, n, line = [], 0, len(text), 1
+    while i < n:
+        c = text[i]
+        if c == "\n":
+            line += 1
+            i += 1
+            continue
+        if c.isspace():
+            i += 1
+            continue
+        if c == '"':
+            j = i + 1
+            while j < n and text[j] != '"':
+                j += 2 if text[j] == "\\" else 1
+            toks.append(Tok("str", text[i:j + 1], text[i + 1:j], line))
+            i = j + 1
+            continue
+        if c.isdigit() or (c == "." and i + 1 < n and text[i + 1].isdigit()):
+            m = _num.match(text, i)
+            mant, exp, suf = m.group(1), m.group(2) or "", m.group(3)
+            is_int = "." not in mant and not exp and not suf
+            v = int(mant) if is_int else float(mant + exp) * (SI[suf] if suf else 1.0)
+            toks.append(Tok("num", m.group(0), v, line))
+            i = m.end()
+            continue
+        if c == "\\":   # escaped identifier: up to the next whitespace
+            j = i + 1
+            while j < n and not text[j].isspace():
+                j += 1
+            toks.append(Tok("id", text[i + 1:j], None, line))
+            i = j
+            continue
+        if c == "$":
+            m = _ident.match(text, i + 1)
+            toks.append(Tok("sys", "$" + m.group(0), None, line))
+            i = m.end()
+            continue
+        m = _ident.match(text, i)
+        if m:
+            toks.append(Tok("id", m.group(0), None, line))
+            i = m.end()
+            continue
+        for op in _ops:
+            if text.startswith(op, i):
+                # "(*" opens an attribute only when not "(*)" of an expression
+                toks.append(Tok("op", op, None, line))
+                i += len(op)
+                break
+        else:
+            raise ParseError(f"line {line}: unexpected character {c!r}")
+    toks.append(Tok("eof", "", None, line))
+    return toks
+
+
+# ---- AST -----------------------------------------------------------------------
+@dataclass
+class Num:
+    v: object
+
+
+@dataclass
+class Var:
+    name: str
+
+
+@dataclass
+class Str:
+    s: str
+
+
+@dataclass
+class Call:
+    name: str
+    args: list
+
+
+@dataclass
+class Probe:
+    kind: str          # 'V' or 'I'
+    a: str
+    b: Optional[str]
+
+
+@dataclass
+class Bin:
+    op: str
+    a: object
+    b: object
+
+
+@dataclass
+class Un:
+    op: str
+    a: object
+
+
+@dataclass
+class Tern:
+    c: object
+    a: object
+    b: object
+
+
+@dataclass
+class Block:
+    name: Optional[str]
+    stmts: list
+
+
+@dataclass
+class If:
+    c: object
+    t: object
+    e: object
+
+
+@dataclass
+class Case:
+    expr: object
+    items: list        # [(labels or None, stmt)]
+
+
+@dataclass
+class For:
+    init: object
+    cond: object
+    step: object
+    body: object
+
+
+@dataclass
+class Assign:
+    name: str
+    expr: object
+
+
+@dataclass
+class Contrib:
+    kind: str
+    a: str
+    b: Optional[str]
+    expr: object
+
+
+@dataclass
+class Task:
+    name: str
+    args: list
+
+
+@dataclass
+class Param:
+    name: str
+    type: str          # 'real' or 'integer'
+    default: object
+    line: int = 0
+
+
+@dataclass
+class Function:
+    name: str
+    rtype: str
+    inputs: List[str]
+    locals: Dict[str, str]
+    body: object
+
+
+@dataclass
+class Module:
+    name: str
+    ports: List[str]
+    nodes: List[str]                   # ports first, then internal nodes
+    params: List[Param] = field(default_factory=list)
+    vars: Dict[str, str] = field(default_factory=dict)      # name -> 'real' / 'integer'
+    functions: Dict[str, Function] = field(default_factory=dict)
+    analog: list = field(default_factory=list)
+
+
+class Parser:
+    def __init__(self, toks: List[Tok]):
+        self.t = toks
+        self.i = 0
+
+    # -- token helpers
+    def peek(self, k=0) -> Tok:
+        return self.t[self.i + k]
+
+    def next(self) -> Tok:
+        tk = self.t[self.i]
+        self.i += 1
+        return tk
+
+    def at(self, text: str) -> bool:
+        tk = self.peek()
+        return tk.kind in ("op", "id") and tk.text == text
+
+    def accept(self, text: str) -> bool:
+        if self.at(text):
+            self.i += 1
+            return True
+        return False
+
+    def expect(self, text: str) -> Tok:
+        tk = self.next()
+        if tk.text != text:
+            raise ParseError(f"line {tk.line}: expected {text!r}, got {tk.text!r}")
+        return tk
+
+    def ident(self) -> str:
+        tk = self.next()
+        if tk.kind != "id":
+            raise ParseError(f"line {tk.line}: expected identifier, got {tk.text!r}")
+        return tk.text
+
+    def skip_attr(self):
+        while self.at("(*"):
+            while not self.accept("*)"):
+                self.next()
+
+    # -- module
+    def module(self) -> Module:
+        while not self.at("module"):
+            if self.peek().kind == "eof":
+                raise ParseError("no module")
+            self.skip_decl()
+        self.expect("module")
+        name = self.ident()
+        ports = []
+        if self.accept("("):
+            if not self.at(")"):
+                ports.append(self.ident())
+                while self.accept(","):
+                    ports.append(self.ident())
+            self.expect(")")
+        self.expect(";")
+        mod = Module(name, ports, list(ports))
+        while not self.accept("endmodule"):
+            self.item(mod)
+        return mod
+
+    def skip_decl(self):
+        # discipline / nature declarations before the module
+        tk = self.next()
+        if tk.text in ("discipline", "nature"):
+            end = "enddiscipline" if tk.text == "discipline" else "endnature"
+            while not self.accept(end):
+                self.next()
+
+    def item(self, mod: Module):
+        self.skip_attr()
+        tk = self.peek()
+        t = tk.text
+        if t in ("inout", "input", "output"):
+            self.next()
+            self.names()
+            self.expect(";")
+        elif t in ("electrical", "thermal", "voltage", "current", "ground"):
+            self.next()
+            for nm in self.names():
+                if nm not in mod.nodes:
+                    mod.nodes.append(nm)
+            self.expect(";")
+        elif t == "branch":
+            while not self.accept(";"):
+                self.next()
+        elif t in ("parameter", "localparam"):
+            self.next()
+            self.param_decl(mod)
+        elif t in ("real", "integer"):
+            self.next()
+            for nm in self.var_names():
+                mod.vars[nm] = t
+            self.expect(";")
+        elif t == "analog":
+            self.next()
+            if self.accept("function"):
+                self.function(mod)
+            else:
+                mod.analog.append(self.stmt(mod))
+        elif t == "genvar":
+            while not self.accept(";"):
+                self.next()
+        else:
+            raise ParseError(f"line {tk.line}: unexpected {t!r} in module")
+
+    def names(self) -> List[str]:
+        out = [self.ident()]
+        while self.accept(","):
+            out.append(self.ident())
+        return out
+
+    def var_names(self) -> List[str]:
+        out = []
+        while True:
+            nm = self.ident()
+            if self.accept("["):         # arrays: not used by the supported models
+                raise ParseError(f"array variable {nm} not supported")
+            if self.accept("="):
+                self.expr()
+            out.append(nm)
+            if not self.accept(","):
+                return out
+
+    def param_decl(self, mod: Module):
+        typ = "real"
+        if self.at("real") or self.at("integer"):
+            typ = self.next().text
+        while True:
+            line = self.peek().line
+            nm = self.ident()
+            self.expect("=")
+            dflt = self.expr()
+            # range constraints: from [a:b] / (a:b) / exclude ...
+            while self.at("from") or self.at("exclude"):
+                self.next()
+                if self.at("[") or self.at("("):
+                    self.next()
+                    depth = 1
+                    while depth:
+                        tk = self.next()
+                        if tk.text in ("[", "("):
+                            depth += 1
+                        elif tk.text in ("]", ")"):
+                            depth -= 1
+                else:
+                    self.expr()
+            mod.params.append(Param(nm, typ, dflt, line))
+            if not self.accept(","):
+                break
+        self.skip_attr()
+        self.expect(";")
+
+    def function(self, mod: Module):
+        rtype = "real"
+        if self.at("real") or self.at("integer"):
+            rtype = self.next().text
+        name = self.ident()
+        self.expect(";")
+        inputs: List[str] = []
+        locs: Dict[str, str] = {}
+        while True:
+            if self.accept("input") or self.accept("output") or self.accept("inout"):
+                inputs += self.names()
+                self.expect(";")
+            elif self.at("real") or self.at("integer"):
+                typ = self.next().text
+                for nm in self.names():
+                    locs[nm] = typ
+                self.expect(";")
+            else:
+                break
+        body = self.stmt(mod)
+        self.expect("endfunction")
+        mod.functions[name] = Function(name, rtype, inputs, locs, body)
+
+    # -- statements
+    def stmt(self, mod: Module):
+        self.skip_attr()
+        tk = self.peek()
+        t = tk.text
+        if t == "begin":
+            self.next()
+            name = None
+            if self.accept(":"):
+                name = self.ident()
+            stmts = []
+            while not self.accept("end"):
+                self.skip_attr()
+                if self.at("real") or self.at("integer"):
+                    typ = self.next().text
+                    for nm in self.var_names():
+                        if mod.vars.get(nm, typ) != typ:
+                            raise ParseError(f"{nm} declared as both real and integer")
+                        mod.vars[nm] = typ
+                    self.expect(";")
+                    continue
+                stmts.append(self.stmt(mod))
+            return Block(name, stmts)
+        if t == "if":
+            self.next()
+            self.expect("(")
+            c = self.expr()
+            self.expect(")")
+            th = self.stmt(mod)
+            el = self.stmt(mod) if self.accept("else") else None
+            return If(c, th, el)
+        if t == "case":
+            self.next()
+            self.expect("(")
+            e = self.expr()
+            self.expect(")")
+            items = []
+            while not self.accept("endcase"):
+                if self.accept("default"):
+                    self.accept(":")
+                    items.append((None, self.stmt(mod)))
+                else:
+                    labels = [self.expr()]
+                    while self.accept(","):
+                        labels.append(self.expr())
+                    self.expect(":")
+                    items.append((labels, self.stmt(mod)))
+            return Case(e, items)
+        if t == "for":
+            self.next()
+            self.expect("(")
+            init = self.assign_only(mod)
+            self.expect(";")
+            cond = self.expr()
+            self.expect(";")
+            step = self.assign_only(mod)
+            self.expect(")")
+            return For(init, cond, step, self.stmt(mod))
+        if t == "@":
+            self.next()
+            self.expect("(")
+            depth = 1
+            while depth:
+                tk2 = self.next()
+                if tk2.text == "(":
+                    depth += 1
+                elif tk2.text == ")":
+                    depth -= 1
+            return self.stmt(mod)
+        if tk.kind == "sys":
+            self.next()
+            args = []
+            if self.accept("("):
+                if not self.at(")"):
+                    args.append(self.expr())
+                    while self.accept(","):
+                        args.append(self.expr())
+                self.expect(")")
+            self.expect(";")
+            return Task(t, args)
+        if t == ";":
+            self.next()
+            return Block(None, [])
+        if tk.kind == "id" and self.peek(1).text == "(" and t in ("I", "V"):
+            # contribution I(a,b) <+ expr  (or a function-call statement)
+            save = self.i
+            self.next()
+            self.expect("(")
+            a = self.ident()
+            b = self.ident() if self.accept(",") else None
+            self.expect(")")
+            if self.accept("<+"):
+                e = self.expr()
+                self.expect(";")
+                return Contrib(t, a, b, e)
+            self.i = save
+        s = self.assign_only(mod)
+        self.expect(";")
+        return s
+
+    def assign_only(self, mod):
+        name = self.ident()
+        self.expect("=")
+        return Assign(name, self.expr())
+
+    # -- expressions
+    def expr(self):
+        c = self.bin_expr(0)
+        if self.accept("?"):
+            a = self.expr()
+            self.expect(":")
+            b = self.expr()
+            return Tern(c, a, b)
+        return c
+
+    PREC = [["||"], ["&&"], ["|"], ["^"], ["&"], ["==", "!="], ["<", "<=", ">", ">="],
+            ["<<", ">>"], ["+", "-"], ["*", "/", "%"], ["**"]]
+
+    def bin_expr(self, lvl):
+        if lvl == len(self.PREC):
+            return self.unary()
+        a = self.bin_expr(lvl + 1)
+        ops = self.PREC[lvl]
+        while self.peek().kind == "op" and self.peek().text in ops:
+            op = self.next().text
+            if op == "**":   # right associative
+                b = self.bin_expr(lvl)
+                a = Bin(op, a, b)
+                break
+            b = self.bin_expr(lvl + 1)
+            a = Bin(op, a, b)
+        return a
+
+    def unary(self):
+        if self.peek().kind == "op" and self.peek().text in ("-", "+", "!", "~"):
+            op = self.next().text
+            return Un(op, self.unary())
+        return self.primary()
+
+    def primary(self):
+        tk = self.next()
+        if tk.kind == "num":
+            return Num(tk.val)
+        if tk.kind == "str":
+            return Str(tk.val)
+        if tk.text == "(":
+            e = self.expr()
+            self.expect(")")
+            return e
+        if tk.kind in ("id", "sys"):
+            name = tk.text
+            if self.accept("("):
+                args = []
+                if name in ("V", "I"):
+                    a = self.ident()
+                    b = self.ident() if self.accept(",") else None
+                    self.expect(")")
+                    return Probe(name, a, b)
+                if not self.at(")"):
+                    args.append(self.expr())
+                    while self.accept(","):
+                        args.append(self.expr())
+                self.expect(")")
+                return Call(name, args)
+            return Var(name)
+        raise ParseError(f"line {tk.line}: unexpected {tk.text!r} in expression")
+
+
+def parse_module(text: str) -> Module:
+    return Parser(lex(text)).module()

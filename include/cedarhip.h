@@ -247,6 +247,26 @@ int cs_batch_traffic(cs_ctx* ctx, int32_t bid, double* alg_bytes,
  * launches one run makes. */
 int cs_batch_kernel(cs_ctx* ctx, int32_t bid, int32_t* kernel, int32_t* n_launches);
 
+/* ---- compact-model device evaluation: BSIM-CMG 107 -------------------- *
+ * The model code is generated from the model's Verilog-A source at build
+ * time (the reference compiles the same file through make_spice_device,
+ * src/vasim.jl:663-875).  One lane per instance, FP64, partial derivatives
+ * by forward-mode dual numbers.  Per instance: the internal nodes di, si
+ * are solved for resistive KCL, then the six-node stamps are condensed onto
+ * the terminals d, g, s, e.  out[40 per instance] = I[4], Q[4], G[16],
+ * C[16] (row-major, row = terminal equation, column = terminal voltage).
+ * cs_bsimcmg_available: 1 when the library was built with the model.
+ * cs_bsimcmg_param: set a model-card or instance-template parameter by its
+ * Verilog-A name (applies to later evaluations; CS_E_INVALID if unknown).
+ * cs_bsimcmg_eval: v is [n][4] terminal voltages; temp_c in Celsius;
+ * iters[n] (may be NULL) the internal-node Newton iterations or -1;
+ * kernel_ms the device time of the evaluation alone (inputs resident). */
+int cs_bsimcmg_available(void);
+int cs_bsimcmg_param(cs_ctx* ctx, const char* name, double value);
+int cs_bsimcmg_eval(cs_ctx* ctx, int64_t n, const double* L, const double* nfin,
+                    const double* v, double temp_c, double* out, int32_t* iters,
+                    double* kernel_ms);
+
 /* ---- batched adjoint sensitivities ------------------------------------- *
  * Replaces forward sensitivity through DAECompiler's paramjac
  * (ODEForwardSensitivityProblem, test/sensitivity.jl:29-41) for the energy
