@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--kernel", choices=["auto", "lds", "global"], default="auto",
                     help="batched kernel: LDS-resident plan (auto when it fits) or global plan")
+    ap.add_argument("--bsim-inst", type=int, default=1 << 20,
+                    help="BSIM-CMG device-eval microbenchmark size (0: skip)")
+    ap.add_argument("--bsim-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -65,6 +68,11 @@ def main():
     from cedarhip import lib as L
     from cedarhip.workloads import ring_mc, ring_widths
 
+    if args.bsim_only:
+        eng = ch.Engine(0)
+        print(json.dumps({"device_eval": bsim_section(ch, args, eng)}), flush=True)
+        eng.close()
+        return
     if args.adjoint_only:
         eng = ch.Engine(0)
         print(json.dumps({"adjoint": adjoint_section(ch, L, ring_widths, args, eng)}), flush=True)
@@ -180,6 +188,8 @@ def main():
         log(f"cpu: {k / dt:.3f} inst/s ({dt:.1f} s, {thr} threads, rc {rc})")
     if args.adjoint_inst > 0:
         out["adjoint"] = adjoint_section(ch, L, ring_widths, args, eng)
+    if args.bsim_inst > 0:
+        out["device_eval"] = bsim_section(ch, args, eng)
     eng.close()
     print(json.dumps(out), flush=True)
 
@@ -210,6 +220,57 @@ def adjoint_section(ch, L, ring_widths, args, eng):
             "mean_steps": float(res.n_steps.mean()),
             "workload": "51-stage ring, W~U[0.5,4]um (102 widths), supply energy 0-5 ns, "
                         "dE/dW by discrete adjoint, checkpoint every 64 steps"}
+
+
+def bsim_section(ch, args, eng):
+    """Device-eval microbenchmark: n BSIM-CMG 107 instances at the default
+    card, L ~ U[20,100] nm, NFIN ~ U{1..8}, Vgs ~ U[-0.2,1] V, Vds ~ U[0,1] V,
+    Vbs = 0; outputs I, Q (4) and the condensed 4x4 G, C per instance.  The
+    value is evaluations/s of the kernel (inputs resident in HBM, HIP events
+    around the launch); the wall rate including the PCIe copies is reported
+    beside it."""
+    if not ch.bsimcmg_available(eng):
+        log("BSIM-CMG not in this build")
+        return {"error": "library built without the generated BSIM-CMG model"}
+    n = args.bsim_inst
+    rng = np.random.default_rng(args.seed)
+    Lg = rng.uniform(20e-9, 100e-9, n)
+    nf = rng.integers(1, 9, n).astype(np.float64)
+    v = np.zeros((n, 4))
+    v[:, 0] = rng.uniform(0.0, 1.0, n)
+    v[:, 1] = rng.uniform(-0.2, 1.0, n)
+    log(f"bsim: {n} instances")
+    r = ch.bsimcmg_eval(Lg, nf, v, engine=eng)            # warmup (module load, card upload)
+    kms, walls = [], []
+    for k in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        r = ch.bsimcmg_eval(Lg, nf, v, engine=eng)
+        walls.append(time.perf_counter() - t0)
+        kms.append(r.kernel_ms)
+        log(f"bsim step {k}: kernel {r.kernel_ms:.2f} ms, wall {walls[-1] * 1e3:.1f} ms")
+    kernel_ms = float(np.mean(kms))
+    evals = int(np.sum(r.iters + 1))                     # model evaluations incl. internal Newton
+    out = {"value": n / (kernel_ms * 1e-3), "unit": "device evals/s", "n": n,
+           "kernel": "k_bsimcmg", "kernel_ms": kernel_ms,
+           "wall_incl_pcie_evals_per_s": n / float(np.mean(walls)),
+           "model_evals_per_instance": evals / n,
+           "failed": int(np.sum(r.iters <= 0)),
+           "workload": "BSIM-CMG 107 default card, L~U[20,100]nm, NFIN~U{1..8}, "
+                       "Vgs~U[-0.2,1], Vds~U[0,1], Vbs=0, FP64, terminal I,Q,G(4x4),C(4x4)"}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        if O.va_lib() is not None:
+            k = 4096
+            t0 = time.perf_counter()
+            O.bsim_terminal(Lg[:k], nf[:k], v[:k])
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": k / dt, "unit": "device evals/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"first {k} instances of the same seeded set; the "
+                                             "generated model compiled by g++ -O1 for the host, "
+                                             "internal-node Newton and condensation in numpy"}
+            log(f"bsim cpu: {k / dt:.0f} evals/s")
+    return out
 
 
 if __name__ == "__main__":

@@ -8,8 +8,9 @@ from .netlist import (C, CedarError, CurrentSource, FlatCircuit, Gnd, I, L, Name
                       SimpleCapacitor, SimpleInductor, SimpleResistor, SquareLawMOSFET,
                       UnimplementedDevice, V, VoltageSource, Waveform, lower, net, pulse, pwl,
                       spsin, vccs, vcvs)
-from .engine import (AdjointResult, Batch, BatchResult, adjoint_batch, CedarHIPError, CircuitSweep, CircuitSystem, DCSolution,
-                     Engine, Solution, TranOptions, dc, default_engine, grid_options, tran,
+from .engine import (AdjointResult, Batch, BatchResult, BsimResult, CedarHIPError, CircuitSweep,
+                     CircuitSystem, DCSolution, Engine, Solution, TranOptions, adjoint_batch,
+                     bsimcmg_available, bsimcmg_eval, dc, default_engine, grid_options, tran,
                      tran_batch)
 from . import lib
 

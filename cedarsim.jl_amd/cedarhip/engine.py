@@ -453,3 +453,48 @@ class CircuitSweep:
             return b.results(ms)
         finally:
             b.close()
+
+
+class BsimResult:
+    """Terminal outputs of batched BSIM-CMG evaluations (terminals d, g, s, e):
+    I (n,4) currents, Q (n,4) charges, G (n,4,4) = dI/dV, C (n,4,4) = dQ/dV,
+    iters (n,) internal-node Newton iterations (-1: not converged)."""
+
+    def __init__(self, out: np.ndarray, iters: np.ndarray, kernel_ms: float):
+        n = out.shape[0]
+        self.I = out[:, 0:4]
+        self.Q = out[:, 4:8]
+        self.G = out[:, 8:24].reshape(n, 4, 4)
+        self.C = out[:, 24:40].reshape(n, 4, 4)
+        self.iters = iters
+        self.kernel_ms = kernel_ms
+
+
+def bsimcmg_available(engine: Engine = None) -> bool:
+    eng = engine or default_engine()
+    return bool(eng.lib.cs_bsimcmg_available())
+
+
+def bsimcmg_eval(L_gate, nfin, v, temp_c: float = 27.0, params: Dict[str, float] = None,
+                 engine: Engine = None) -> BsimResult:
+    """Evaluate BSIM-CMG 107 instances on the device (one lane per instance).
+    L_gate, nfin: per-instance gate length [m] and fin count; v: (n,4)
+    terminal voltages (d, g, s, e); params: model-card or instance-template
+    parameters by their Verilog-A names (default card otherwise).  The model
+    is compiled from its Verilog-A source, as make_spice_device does
+    (src/vasim.jl:663-875)."""
+    eng = engine or default_engine()
+    for k, val in (params or {}).items():
+        eng.check(eng.lib.cs_bsimcmg_param(eng.ctx, k.encode(), float(val)))
+    Lg = np.ascontiguousarray(L_gate, np.float64)
+    nf = np.ascontiguousarray(nfin, np.float64)
+    vv = np.ascontiguousarray(v, np.float64).reshape(-1, 4)
+    n = len(Lg)
+    if len(nf) != n or vv.shape[0] != n:
+        raise ValueError("L_gate, nfin and v must describe the same number of instances")
+    out = np.zeros((n, 40))
+    it = np.zeros(n, np.int32)
+    ms = c_double()
+    eng.check(eng.lib.cs_bsimcmg_eval(eng.ctx, n, L.dptr(Lg), L.dptr(nf), L.dptr(vv), float(temp_c),
+                                      L.dptr(out), L.iptr(it), byref(ms)))
+    return BsimResult(out, it, ms.value)

@@ -331,3 +331,90 @@ def tran_batch(flats, params, u0s, o=None, probes=None, nthreads=0, **kw):
     waves = [wave[w_off[i]:w_off[i] + w_sizes[i]].reshape(oo.n_out, len(prl[i])) for i in range(n_inst)]
     ufs = [uf[u_off[i]:u_off[i] + nls[i].n] for i in range(n_inst)]
     return waves, ufs, list(st), rc
+
+
+# ---- BSIM-CMG (liboracle_va.so; the model code vagen generates) ---------------
+_va = None
+
+
+def va_lib():
+    global _va
+    if _va is None:
+        p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "build", "liboracle_va.so")
+        if not os.path.exists(p):
+            return None
+        lib_ = ctypes.CDLL(p)
+        lib_.or_bsim_raw.restype = c_int
+        lib_.or_bsim_raw.argtypes = [c_longlong, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                     c_double, c_int, POINTER(ctypes.c_char_p), POINTER(c_double), c_int,
+                                     POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                     POINTER(c_double)]
+        _va = lib_
+    return _va
+
+
+def bsim_raw(L_gate, nfin, v6, temp_c=27.0, params=None, derivs=True):
+    """Six-node BSIM-CMG evaluation at given node voltages (d, g, s, e, di, si):
+    I (n,6), Q (n,6) and, with derivs, G = dI/dV, C = dQ/dV (n,6,6)."""
+    lib_ = va_lib()
+    Lg = np.ascontiguousarray(L_gate, np.float64)
+    nf = np.ascontiguousarray(nfin, np.float64)
+    vv = np.ascontiguousarray(v6, np.float64).reshape(-1, 6)
+    n = len(Lg)
+    params = params or {}
+    names = (ctypes.c_char_p * max(len(params), 1))(*[k.encode() for k in params])
+    vals = np.array(list(params.values()) or [0.0], np.float64)
+    I6 = np.zeros((n, 6))
+    Q6 = np.zeros((n, 6))
+    G = np.zeros((n, 6, 6))
+    C = np.zeros((n, 6, 6))
+    rc = lib_.or_bsim_raw(n, _dp(Lg), _dp(nf), _dp(vv), float(temp_c), len(params), names, _dp(vals),
+                          1 if derivs else 0, _dp(I6), _dp(Q6), _dp(G), _dp(C))
+    if rc != 0:
+        raise ValueError("unknown BSIM-CMG parameter")
+    return I6, Q6, G, C
+
+
+def bsim_terminal(L_gate, nfin, v4, temp_c=27.0, params=None, maxit=20, full=False):
+    """Terminal form of BSIM-CMG: solve the internal nodes di, si for their
+    resistive KCL (Newton on the 2x2 block of dI/dV by Cramer's rule, step
+    accepted when |dx| <= 1e-15 (1 + |V_di| + |V_si|), then one more
+    evaluation), and condense G and C onto d, g, s, e by the Schur complement:
+    S = -G_ii^-1 G_it, M = -G_ti G_ii^-1, G' = G_tt + G_ti S,
+    Q' = Q_t + M Q_i, C' = C_tt + C_ti S + M (C_it + C_ii S).
+    Returns I, Q (n,4), G, C (n,4,4) and the Newton step counts (-1: not
+    converged within maxit); with full, also the solved six-node voltages and
+    the six-node I, Q, G, C there."""
+    v4 = np.ascontiguousarray(v4, np.float64).reshape(-1, 4)
+    n = v4.shape[0]
+    v6 = np.concatenate([v4, v4[:, [0]], v4[:, [2]]], axis=1)
+    it = np.zeros(n, np.int32)
+    conv = np.zeros(n, bool)
+    for _ in range(maxit + 1):
+        I6, Q6, G, C = bsim_raw(L_gate, nfin, v6, temp_c, params)
+        if conv.all():
+            break
+        a, b = G[:, 4, 4], G[:, 4, 5]
+        c, d = G[:, 5, 4], G[:, 5, 5]
+        f0, f1 = I6[:, 4], I6[:, 5]
+        with np.errstate(all="ignore"):
+            det = a * d - b * c
+            dx0 = (d * f0 - b * f1) / det
+            dx1 = (a * f1 - c * f0) / det
+        act = ~conv
+        v6[act, 4] -= dx0[act]
+        v6[act, 5] -= dx1[act]
+        it[act] += 1
+        tol = 1e-15 * (1.0 + np.abs(v6[:, 4]) + np.abs(v6[:, 5]))
+        conv |= act & (np.abs(dx0) <= tol) & (np.abs(dx1) <= tol)
+    it[~conv] = -1
+    Gii = G[:, 4:6, 4:6]
+    S = -np.linalg.solve(Gii, G[:, 4:6, 0:4])
+    M = -np.transpose(np.linalg.solve(np.transpose(Gii, (0, 2, 1)),
+                                      np.transpose(G[:, 0:4, 4:6], (0, 2, 1))), (0, 2, 1))
+    Gt = G[:, 0:4, 0:4] + G[:, 0:4, 4:6] @ S
+    Ct = C[:, 0:4, 0:4] + C[:, 0:4, 4:6] @ S + M @ (C[:, 4:6, 0:4] + C[:, 4:6, 4:6] @ S)
+    Qt = Q6[:, 0:4] + (M @ Q6[:, 4:6, None])[:, :, 0]
+    if full:
+        return I6[:, 0:4], Qt, Gt, Ct, it, (v6, I6, Q6, G, C)
+    return I6[:, 0:4], Qt, Gt, Ct, it

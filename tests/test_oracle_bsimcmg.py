@@ -1,0 +1,158 @@
+"""Pins the BSIM-CMG oracle (the model header vagen generates from the bundled
+Verilog-A source, evaluated on the host; the internal-node solve and terminal
+condensation restated in numpy).
+
+The reference holds no numeric BSIM-CMG outputs, so the checks come from
+what the mathematics fixes:
+  * dual-number partials against central finite differences of the model;
+  * current and charge conservation over the six nodes and the four terminals;
+  * invariance to a common shift of all node voltages (row sums of G, C vanish);
+  * Ids = 0 at Vds = 0 and source/drain exchange symmetry of the model;
+  * the condensed conductance against finite differences of the terminal
+    currents (internal nodes re-solved at each perturbed point);
+  * the condensed capacitance against the low-frequency limit of the complex
+    Schur complement Y(jw) = Y_tt - Y_ti Y_ii^-1 Y_it, Im(Y)/w as w -> 0.
+A dropped term, a transposed block or a wrong node index fails one of them.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.skipif(O.va_lib() is None,
+                                reason="liboracle_va.so not built (the model header is generated "
+                                       "from the bundled Verilog-A source at build time)")
+
+
+def c1_sample(n, seed=1):
+    """Instances drawn as in the device-eval workload: L ~ U[20,100] nm,
+    NFIN ~ U{1..8}, Vgs ~ U[-0.2,1], Vds ~ U[0,1], Vs = Ve = 0."""
+    rng = np.random.default_rng(seed)
+    L = rng.uniform(20e-9, 100e-9, n)
+    nf = rng.integers(1, 9, n).astype(np.float64)
+    v = np.zeros((n, 4))
+    v[:, 0] = rng.uniform(0.0, 1.0, n)
+    v[:, 1] = rng.uniform(-0.2, 1.0, n)
+    return L, nf, v
+
+
+def six(v4, dv_int=None):
+    v6 = np.concatenate([v4, v4[:, [0]], v4[:, [2]]], axis=1)
+    if dv_int is not None:
+        v6[:, 4:6] += dv_int
+    return v6
+
+
+def test_partials_match_central_differences():
+    n = 24
+    L, nf, v = c1_sample(n, 2)
+    rng = np.random.default_rng(3)
+    v6 = six(v, rng.uniform(-0.02, 0.02, (n, 2)))     # off the internal solution too
+    I, Q, G, C = O.bsim_raw(L, nf, v6)
+    h = 1e-6
+    for j in range(6):
+        vp, vm = v6.copy(), v6.copy()
+        vp[:, j] += h
+        vm[:, j] -= h
+        Ip, Qp, _, _ = O.bsim_raw(L, nf, vp, derivs=False)
+        Im, Qm, _, _ = O.bsim_raw(L, nf, vm, derivs=False)
+        dI = (Ip - Im) / (2 * h)
+        dQ = (Qp - Qm) / (2 * h)
+        gs = np.abs(G).max(axis=(1, 2))[:, None]
+        cs = np.abs(C).max(axis=(1, 2))[:, None]
+        assert np.all(np.abs(dI - G[:, :, j]) <= 1e-6 * gs + 1e-5 * np.abs(G[:, :, j])), j
+        assert np.all(np.abs(dQ - C[:, :, j]) <= 1e-6 * cs + 1e-5 * np.abs(C[:, :, j])), j
+
+
+def test_raw_conservation_and_shift_invariance():
+    n = 64
+    L, nf, v = c1_sample(n, 4)
+    v6 = six(v, np.random.default_rng(5).uniform(-0.01, 0.01, (n, 2)))
+    I, Q, G, C = O.bsim_raw(L, nf, v6)
+    isc = np.abs(I).max(axis=1)
+    qsc = np.abs(Q).max(axis=1)
+    assert np.all(np.abs(I.sum(axis=1)) <= 1e-12 * isc + 1e-20)
+    assert np.all(np.abs(Q.sum(axis=1)) <= 1e-12 * qsc + 1e-30)
+    # a common shift of all six voltages changes nothing: rows sum to zero
+    gsc = np.abs(G).max(axis=(1, 2))
+    csc = np.abs(C).max(axis=(1, 2))
+    assert np.all(np.abs(G.sum(axis=2)).max(axis=1) <= 1e-12 * gsc)
+    assert np.all(np.abs(C.sum(axis=2)).max(axis=1) <= 1e-12 * csc + 1e-30)
+    # and each column's currents sum to zero (KCL for every perturbation)
+    assert np.all(np.abs(G.sum(axis=1)).max(axis=1) <= 1e-12 * gsc)
+
+
+def test_zero_vds_and_source_drain_exchange():
+    n = 32
+    L, nf, v = c1_sample(n, 6)
+    v[:, 0] = 0.0
+    I, Q, G, C, it = O.bsim_terminal(L, nf, v)
+    assert np.all(it > 0)
+    gm = np.abs(G).max(axis=(1, 2))
+    assert np.all(np.abs(I[:, 0]) <= 1e-12 * gm + 1e-18)
+    # exchange d and s: Vd'=Vs, Vs'=Vd, raised gate so both orientations conduct
+    L, nf, v = c1_sample(n, 7)
+    v[:, 2] = 0.3
+    w = v[:, [2, 1, 0, 3]]
+    I1, Q1, _, _, _ = O.bsim_terminal(L, nf, v)
+    I2, Q2, _, _, _ = O.bsim_terminal(L, nf, w)
+    isc = np.abs(I1).max(axis=1, keepdims=True)
+    qsc = np.abs(Q1).max(axis=1, keepdims=True)
+    floor = 1e-12                                   # internal-node rounding floor, amperes
+    assert np.all(np.abs(I2[:, [2, 1, 0, 3]] - I1) <= 1e-9 * isc + floor)
+    # the gate and bulk charges are exchange-symmetric; the model stamps the
+    # body charge qb on si in both orientations (qd/qs swap, then
+    # I(di,si) <+ ddt(qd), I(gi,si) <+ ddt(qg)), so only qd + qs is
+    assert np.all(np.abs(Q2[:, 1] - Q1[:, 1]) <= 1e-9 * qsc[:, 0])
+    assert np.all(np.abs(Q2[:, 3] - Q1[:, 3]) <= 1e-9 * qsc[:, 0])
+    assert np.all(np.abs((Q2[:, 0] + Q2[:, 2]) - (Q1[:, 0] + Q1[:, 2])) <= 1e-9 * qsc[:, 0])
+
+
+def test_terminal_conductance_matches_fd_of_solved_currents():
+    n = 16
+    L, nf, v = c1_sample(n, 8)
+    v[:, 0] = np.maximum(v[:, 0], 0.2)
+    v[:, 1] = np.maximum(v[:, 1], 0.45)               # on: currents well above the floor
+    I, Q, G, C, it, (_, _, _, G6, _) = O.bsim_terminal(L, nf, v, full=True)
+    assert np.all(it > 0)
+    # the drain current is resolved to ~1e-13 A (rounding of V_d - V_di over
+    # the 1e-3 ohm series resistor), so h trades that noise against the
+    # O(h^2) truncation
+    h = 2e-4
+    for j in range(4):
+        vp, vm = v.copy(), v.copy()
+        vp[:, j] += h
+        vm[:, j] -= h
+        Ip = O.bsim_terminal(L, nf, vp)[0]
+        Im = O.bsim_terminal(L, nf, vm)[0]
+        dI = (Ip - Im) / (2 * h)
+        gs = np.abs(G).max(axis=(1, 2))[:, None]
+        assert np.all(np.abs(dI - G[:, :, j]) <= 1e-5 * gs), j
+    # conservation of the condensed stamps, to the rounding of the series
+    # conductances (1e3 S) that the Schur complement cancels
+    gsc = np.abs(G).max(axis=(1, 2))
+    gint = np.abs(G6[:, 4:6, 4:6]).max(axis=(1, 2))
+    eps = np.finfo(np.float64).eps
+    assert np.all(np.abs(G.sum(axis=1)).max(axis=1) <= 1e-12 * gsc + 16 * eps * gint)
+    assert np.all(np.abs(G.sum(axis=2)).max(axis=1) <= 1e-12 * gsc + 16 * eps * gint)
+    qsc = np.abs(Q).max(axis=1)
+    assert np.all(np.abs(Q.sum(axis=1)) <= 1e-12 * qsc)
+
+
+def test_terminal_capacitance_is_low_frequency_admittance():
+    n = 16
+    L, nf, v = c1_sample(n, 9)
+    I, Q, G, C, it, (v6, I6, Q6, G6, C6) = O.bsim_terminal(L, nf, v, full=True)
+    assert np.all(it > 0)
+    assert np.all(np.abs(I6[:, 4:6]) <= 1e-12)          # internal KCL holds
+    w = 1e3                                          # rad/s: w*C << G by many decades
+    Y = G6 + 1j * w * C6
+    Yr = Y[:, 0:4, 0:4] - Y[:, 0:4, 4:6] @ np.linalg.solve(Y[:, 4:6, 4:6], Y[:, 4:6, 0:4])
+    csc = np.abs(C).max(axis=(1, 2))[:, None, None]
+    gsc = np.abs(G).max(axis=(1, 2))[:, None, None]
+    gint = np.abs(G6[:, 4:6, 4:6]).max(axis=(1, 2))[:, None, None]
+    eps = np.finfo(np.float64).eps
+    assert np.all(np.abs(Yr.real - G) <= 1e-12 * gsc + 16 * eps * gint)
+    assert np.all(np.abs(Yr.imag / w - C) <= 1e-6 * csc)
+    # the condensed charges conserve
+    assert np.all(np.abs(C.sum(axis=1)).max(axis=1) <= 1e-9 * csc[:, 0, 0])
