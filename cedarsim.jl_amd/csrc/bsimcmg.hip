@@ -10,7 +10,7 @@
 //
 // Per instance:
 //   1. the internal nodes di, si are solved so that their resistive KCL
-//      holds (Newton on the 2x2 block of dI/dV);
+//      holds (Newton on the 2x2 block of dI/dV, partials along di, si only);
 //   2. the six-node conductance G = dI/dV and capacitance C = dQ/dV are
 //      condensed onto the four terminals (Schur complement over di, si):
 //         S = -G_ii^-1 G_it   (dV_i/dV_t),   M = -G_ti G_ii^-1
@@ -26,74 +26,15 @@
 #include <cstring>
 
 #include "kernels.h"
-#include "va_runtime.h"
-#include "../_gen/bsimcmg.h"
+#include "bsim_terminal.h"
 
 namespace cedar {
 
 namespace {
 
 using namespace va_bsimcmg;
-constexpr int NN = n_nodes;
-static_assert(NN == 6, "BSIM-CMG with internal di, si");
-using D6 = VaDual<NN>;
 constexpr int kBsimThreads = 256;
 constexpr int kOutPer = 40;
-
-}  // namespace
-
-// One instance: returns the Newton iterations used on the internal nodes (or
-// a negative value when they did not converge).
-CEDAR_VA_HD inline int bsimcmg_terminal(const Card& P, const Inst& Ip, double temp,
-                                        const double* vt, double* out) {
-    double V[NN] = {vt[0], vt[1], vt[2], vt[3], vt[0], vt[2]};
-    D6 Vd[NN], I[NN], Q[NN];
-    int it = 0, conv = 0;
-    for (; it < 20; it++) {
-        for (int k = 0; k < NN; k++) Vd[k] = D6::var(V[k], k);
-        eval<D6>(P, Ip, temp, Vd, I, Q);
-        if (conv) break;
-        const double a = I[4].d[4], b = I[4].d[5], c = I[5].d[4], d = I[5].d[5];
-        const double f0 = I[4].v, f1 = I[5].v;
-        const double det = a * d - b * c;
-        const double dx0 = (d * f0 - b * f1) / det, dx1 = (a * f1 - c * f0) / det;
-        V[4] -= dx0;
-        V[5] -= dx1;
-        if (!(dx0 == dx0) || !(dx1 == dx1)) return -1;
-        const double tol = 1e-15 * (1.0 + (V[4] < 0 ? -V[4] : V[4]) + (V[5] < 0 ? -V[5] : V[5]));
-        if ((dx0 < 0 ? -dx0 : dx0) <= tol && (dx1 < 0 ? -dx1 : dx1) <= tol) conv = 1;
-    }
-    if (!conv) return -1;
-    // condensation
-    const double gii[2][2] = {{I[4].d[4], I[4].d[5]}, {I[5].d[4], I[5].d[5]}};
-    const double det = gii[0][0] * gii[1][1] - gii[0][1] * gii[1][0];
-    const double inv[2][2] = {{gii[1][1] / det, -gii[0][1] / det}, {-gii[1][0] / det, gii[0][0] / det}};
-    double S[2][4], M[4][2];
-    for (int j = 0; j < 4; j++)
-        for (int r = 0; r < 2; r++)
-            S[r][j] = -(inv[r][0] * I[4].d[j] + inv[r][1] * I[5].d[j]);
-    for (int t = 0; t < 4; t++)
-        for (int r = 0; r < 2; r++)
-            M[t][r] = -(I[t].d[4] * inv[0][r] + I[t].d[5] * inv[1][r]);
-    double* oI = out;
-    double* oQ = out + 4;
-    double* oG = out + 8;
-    double* oC = out + 24;
-    for (int t = 0; t < 4; t++) {
-        oI[t] = I[t].v;
-        oQ[t] = Q[t].v + M[t][0] * Q[4].v + M[t][1] * Q[5].v;
-        for (int j = 0; j < 4; j++) {
-            oG[4 * t + j] = I[t].d[j] + I[t].d[4] * S[0][j] + I[t].d[5] * S[1][j];
-            const double cij = Q[t].d[j] + Q[t].d[4] * S[0][j] + Q[t].d[5] * S[1][j];
-            const double c4 = Q[4].d[j] + Q[4].d[4] * S[0][j] + Q[4].d[5] * S[1][j];
-            const double c5 = Q[5].d[j] + Q[5].d[4] * S[0][j] + Q[5].d[5] * S[1][j];
-            oC[4 * t + j] = cij + M[t][0] * c4 + M[t][1] * c5;
-        }
-    }
-    return it;
-}
-
-namespace {
 
 __global__ __launch_bounds__(kBsimThreads) void k_bsimcmg(BsimArgs A) {
     __shared__ Card card;

@@ -217,6 +217,8 @@ CEDAR_VA_HD inline VaDual<N> va_exp(const VaDual<N>& x) {
 }
 template <int N>
 CEDAR_VA_HD inline VaDual<N> va_limexp(const VaDual<N>& x) { return va_exp(x); }
+// ln and sqrt partials: the frules of src/va_env.jl:48-55 (dx/x, dx/(2 sqrt x)),
+// with one reciprocal per call instead of a division per partial
 template <int N>
 CEDAR_VA_HD inline VaDual<N> va_ln(const VaDual<N>& x) { return va_chain(x, va_ln(x.v), 1.0 / x.v); }
 template <int N>
@@ -268,16 +270,22 @@ template <int N>
 CEDAR_VA_HD inline VaDual<N> va_ceil(const VaDual<N>& x) { return VaDual<N>(std::ceil(x.v)); }
 template <int N>
 CEDAR_VA_HD inline VaDual<N> va_pow(const VaDual<N>& a, const VaDual<N>& b) {
-    // d(a^b) = b a^(b-1) da + a^b ln(a) db; the db term is dropped when b
-    // carries no derivative (a strong zero, as in src/va_env.jl)
+    // The pow frule of src/va_env.jl:57-67 with ChainRules' partials
+    //   dy/da = b y / a            (a != 0 or b < 0; at a = 0: 1 for b = 1,
+    //                               0 for b = 0 or b > 1, +inf otherwise)
+    //   dy/db = y log|a|           (a != 0 or b < 0; 0 otherwise)
+    // and the db term dropped where b carries no derivative along a partial
+    // (a strong zero), so every partial is independent of the others.
     const double y = va_pow(a.v, b.v);
+    const bool nz = a.v != 0.0 || b.v < 0.0;
+    const double dx = nz ? b.v * y / a.v
+                         : (b.v == 1.0 ? 1.0 : ((b.v == 0.0 || b.v > 1.0) ? 0.0 : (double)INFINITY));
     bool db = false;
     for (int i = 0; i < N; i++) db = db || (b.d[i] != 0.0);
-    const double dx = b.v == 0.0 ? 0.0 : b.v * va_pow(a.v, b.v - 1.0);
-    const double dp = db ? y * va_ln(a.v) : 0.0;
+    const double dp = (db && nz) ? y * std::log(std::fabs(a.v)) : 0.0;
     VaDual<N> r;
     r.v = y;
-    for (int i = 0; i < N; i++) r.d[i] = dx * a.d[i] + (db ? dp * b.d[i] : 0.0);
+    for (int i = 0; i < N; i++) r.d[i] = b.d[i] != 0.0 ? dp * b.d[i] + dx * a.d[i] : dx * a.d[i];
     return r;
 }
 template <int N>
