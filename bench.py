@@ -59,10 +59,26 @@ def main():
     ap.add_argument("--bsim-inst", type=int, default=1 << 20,
                     help="BSIM-CMG device-eval microbenchmark size (0: skip)")
     ap.add_argument("--bsim-only", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only fill and copy a buffer of this many MiB on the device (the known-size "
+                         "kernels that scale the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.copy_calib:
+        import torch
+        nb = args.copy_calib << 20
+        a = torch.empty(nb, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        a.fill_(1)                  # a kernel that writes nb bytes
+        tot = a.sum(dtype=torch.int64)   # a kernel that reads nb bytes
+        b.copy_(a)                  # device copy (may run on a copy engine)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": nb, "ok": bool(b[-1].item() == 1 and int(tot) == nb)}),
+              flush=True)
+        return
 
     import cedarhip as ch
     from cedarhip import lib as L
@@ -147,7 +163,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
-            "kernel": "k_tran_lds" if batch.kernel == L.BATCH_KERNEL_LDS else "k_tran_batch",
+            "kernel": "k_tran_pack" if batch.kernel == L.BATCH_KERNEL_LDS else "k_tran_batch",
             "kernel_ms": kernel_ms,
             "alg_bytes_per_launch": res.alg_bytes,
             "alg_flops_per_launch": res.alg_flops,

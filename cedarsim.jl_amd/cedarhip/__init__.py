@@ -10,7 +10,7 @@ from .netlist import (C, CedarError, CurrentSource, FlatCircuit, Gnd, I, L, Name
                       spsin, vccs, vcvs)
 from .engine import (AdjointResult, Batch, BatchResult, BsimResult, CedarHIPError, CircuitSweep,
                      CircuitSystem, DCSolution, Engine, Solution, TranOptions, adjoint_batch,
-                     bsimcmg_available, bsimcmg_eval, dc, default_engine, grid_options, tran,
+                     big_analyze, bsimcmg_available, bsimcmg_eval, dc, default_engine, grid_options, tran,
                      tran_batch)
 from . import lib
 

@@ -92,9 +92,12 @@ class Engine:
             msg = self.lib.cs_last_error(self.ctx)
             raise CedarHIPError(rc, msg.decode() if msg else "")
 
-    def load(self, circuit) -> "CircuitSystem":
+    def load(self, circuit, solver: int = L.SOLVER_AUTO) -> "CircuitSystem":
+        """Compile a circuit.  solver: SOLVER_AUTO (supernodal multifrontal LU
+        from SUPERNODAL_MIN_N unknowns, the level-scheduled LU below),
+        SOLVER_LEVEL or SOLVER_SUPERNODAL."""
         flat = circuit if isinstance(circuit, FlatCircuit) else lower(circuit)
-        return CircuitSystem(self, flat)
+        return CircuitSystem(self, flat, solver)
 
     def close(self):
         if self.ctx:
@@ -121,18 +124,31 @@ def default_engine() -> Engine:
 class CircuitSystem:
     """The compiled circuit: symbolic plan on the device plus the DAE interface."""
 
-    def __init__(self, engine: Engine, flat: FlatCircuit):
+    def __init__(self, engine: Engine, flat: FlatCircuit, solver: int = L.SOLVER_AUTO):
         self.engine = engine
         self.flat = flat
         self._nl = L.NetlistStruct(flat)
         cid = c_int32()
-        engine.check(engine.lib.cs_circuit_load(engine.ctx, byref(self._nl.struct), byref(cid)))
+        engine.check(engine.lib.cs_circuit_load_solver(engine.ctx, byref(self._nl.struct), int(solver),
+                                                       byref(cid)))
         self.cid = cid.value
         info = L.cs_circuit_info()
         engine.check(engine.lib.cs_circuit_info_get(engine.ctx, self.cid, byref(info)))
         self.info = info
         self.n = info.n
         self.probes = list(range(self.n))
+        big = L.cs_big_info()
+        self.big_info = big if engine.lib.cs_big_info_get(engine.ctx, self.cid, byref(big)) == 0 else None
+
+    @property
+    def supernodal(self) -> bool:
+        return self.big_info is not None
+
+    def lu_times(self):
+        """(factor_ms, solve_ms) of the last lu_refactor / lu_solve (supernodal circuits)."""
+        f, s_ = c_double(), c_double()
+        self.engine.check(self.engine.lib.cs_big_times(self.engine.ctx, self.cid, byref(f), byref(s_)))
+        return f.value, s_.value
 
     # -- structure --------------------------------------------------------
     def params(self) -> np.ndarray:
@@ -453,6 +469,21 @@ class CircuitSweep:
             return b.results(ms)
         finally:
             b.close()
+
+
+def big_analyze(circuit):
+    """Symbolic analysis of the supernodal solver on the host (no GPU): the
+    cs_big_info sizes and the pivot order (rownat, colnat)."""
+    flat = circuit if isinstance(circuit, FlatCircuit) else lower(circuit)
+    lib = L.load_library()
+    nl = L.NetlistStruct(flat)
+    info = L.cs_big_info()
+    rn = np.zeros(flat.n, np.int32)
+    cn = np.zeros(flat.n, np.int32)
+    rc = lib.cs_big_analyze(byref(nl.struct), byref(info), L.iptr(rn), L.iptr(cn))
+    if rc != 0:
+        raise CedarHIPError(rc, "supernodal analysis failed")
+    return info, rn, cn
 
 
 class BsimResult:

@@ -36,6 +36,8 @@ RETCODE_NAMES = {
 MODE_TRAN, MODE_DCOP = 0, 1
 INIT_DCOP, INIT_UIC = 0, 1
 BATCH_KERNEL_AUTO, BATCH_KERNEL_LDS, BATCH_KERNEL_GLOBAL = 0, 1, 2
+SOLVER_AUTO, SOLVER_LEVEL, SOLVER_SUPERNODAL = 0, 1, 2
+SUPERNODAL_MIN_N = 8192
 
 _I = POINTER(c_int32)
 _D = POINTER(c_double)
@@ -73,6 +75,12 @@ class cs_circuit_info(Structure):
                 ("n_probe", c_int32)]
 
 
+class cs_big_info(Structure):
+    _fields_ = [("n", c_int32), ("n_super", c_int32), ("n_levels", c_int32), ("max_front", c_int32),
+                ("max_pivots", c_int32), ("n_hub", c_int32), ("pool", c_int64), ("nnz_factor", c_int64),
+                ("flops", c_double)]
+
+
 class cs_tran_opts(Structure):
     _fields_ = [("t_start", c_double), ("t_stop", c_double), ("reltol", c_double),
                 ("abstol", c_double), ("dt_init", c_double), ("dt_min", c_double),
@@ -101,6 +109,10 @@ SIGNATURES = {
     "cs_analyze": (c_int32, [POINTER(cs_netlist), POINTER(cs_circuit_info), _I, _I, _I, _I]),
     "cs_circuit_load": (c_int32, [c_void_p, POINTER(cs_netlist), _I]),
     "cs_circuit_info_get": (c_int32, [c_void_p, c_int32, POINTER(cs_circuit_info)]),
+    "cs_big_analyze": (c_int32, [POINTER(cs_netlist), POINTER(cs_big_info), _I, _I]),
+    "cs_circuit_load_solver": (c_int32, [c_void_p, POINTER(cs_netlist), c_int32, _I]),
+    "cs_big_info_get": (c_int32, [c_void_p, c_int32, POINTER(cs_big_info)]),
+    "cs_big_times": (c_int32, [c_void_p, c_int32, _D, _D]),
     "cs_circuit_layout": (c_int32, [c_void_p, c_int32, POINTER(cs_param_layout)]),
     "cs_circuit_default_params": (c_int32, [c_void_p, c_int32, _D]),
     "cs_circuit_pattern": (c_int32, [c_void_p, c_int32, _I, _I]),

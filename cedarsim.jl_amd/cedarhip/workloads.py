@@ -14,8 +14,8 @@ from typing import List, Tuple
 
 import numpy as np
 
-from .netlist import (C, FlatCircuit, Gnd, Named, SquareLawMOSFET, V, VoltageSource, lower, net,
-                      pulse)
+from .netlist import (WAVE_DC, WAVE_PULSE, C, FlatCircuit, Gnd, Named, SquareLawMOSFET, V, VoltageSource,
+                      lower, net, pulse)
 
 VDD = 1.8
 NMOS = dict(type=1, vth=0.4, kp=200e-6, w=1e-6, l=0.18e-6)
@@ -113,3 +113,72 @@ def ring_mc(n_inst=4096, seed=0, nmin=11, nmax=101, sigma_vth=0.030
         params.append(p)
         u0s.append(ring_initial_state(f, rng))
     return flats, params, u0s, stages
+
+
+def rc_mesh(nx: int = 450, ny: int = 450, n_inv: int = 10_000, seed: int = 0, vdd: float = 1.0,
+            period: float = 1e-9, tr: float = 50e-12, phase_step: float = 10e-12,
+            rmin: float = 0.5, rmax: float = 2.0, cmin: float = 1e-15, cmax: float = 10e-15) -> FlatCircuit:
+    """Power-grid workload: an nx x ny RC mesh (R ~ U[rmin, rmax] between
+    neighbours, C ~ U[cmin, cmax] to ground at every node) driven by n_inv
+    square-law inverters (the devices of ``inverter``) whose outputs sit on
+    distinct random mesh nodes, each inverter with its own PULSE input
+    (0 -> vdd, rise/fall tr, period ``period``, 50 % duty) at a random phase
+    in [0, period/2) on a ``phase_step`` grid (the grid bounds the number of
+    distinct source breakpoints), VDD ``vdd``.  Built directly as a flat
+    netlist (mesh node (i, j) is unknown i*ny + j)."""
+    rng = np.random.default_rng(seed)
+    nm = nx * ny
+    if n_inv > nm:
+        raise ValueError("more inverters than mesh nodes")
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    node = (ii * ny + jj).astype(np.int32)
+    h = np.stack([node[:, :-1].ravel(), node[:, 1:].ravel()], axis=1)
+    v = np.stack([node[:-1, :].ravel(), node[1:, :].ravel()], axis=1)
+    res_nodes = np.concatenate([h, v]).astype(np.int32)
+    res_r = rng.uniform(rmin, rmax, len(res_nodes))
+    cap_nodes = np.stack([np.arange(nm, dtype=np.int32), np.full(nm, -1, np.int32)], axis=1)
+    cap_c = rng.uniform(cmin, cmax, nm)
+    vdd_node = nm
+    in0 = nm + 1
+    n_nodes = nm + 1 + n_inv
+    out_nodes = rng.choice(nm, n_inv, replace=False).astype(np.int32)
+    nphase = max(1, int(round(0.5 * period / phase_step)))
+    phases = rng.integers(0, nphase, n_inv) * phase_step
+    pw = 0.5 * period - tr
+    # waves: 0 = VDD (DC), 1 + k = PULSE of inverter k
+    wave_kind = np.concatenate([[WAVE_DC], np.full(n_inv, WAVE_PULSE)]).astype(np.int32)
+    wave_len = np.concatenate([[1], np.full(n_inv, 7)]).astype(np.int32)
+    wave_off = np.concatenate([[0], 1 + 7 * np.arange(n_inv)]).astype(np.int32)
+    pdata = np.stack([np.zeros(n_inv), np.full(n_inv, vdd), phases, np.full(n_inv, tr), np.full(n_inv, tr),
+                      np.full(n_inv, pw), np.full(n_inv, period)], axis=1).ravel()
+    wave_data = np.concatenate([[vdd], pdata])
+    wave_dc = np.concatenate([[vdd], np.zeros(n_inv)])
+    vsrc_nodes = np.concatenate([[[vdd_node, -1]],
+                                 np.stack([in0 + np.arange(n_inv), np.full(n_inv, -1)], axis=1)]).astype(np.int32)
+    vsrc_wave = np.arange(1 + n_inv, dtype=np.int32)
+    inp = (in0 + np.arange(n_inv)).astype(np.int32)
+    gnd = np.full(n_inv, -1, np.int32)
+    vd = np.full(n_inv, vdd_node, np.int32)
+    mos_nodes = np.empty((2 * n_inv, 4), np.int32)
+    mos_nodes[0::2] = np.stack([out_nodes, inp, gnd, gnd], axis=1)
+    mos_nodes[1::2] = np.stack([out_nodes, inp, vd, vd], axis=1)
+    nt = len(mos_nodes)
+    mos_type = np.tile(np.array([1, -1], np.int32), n_inv)
+    mos_vth = np.tile([NMOS["vth"], PMOS["vth"]], n_inv).astype(np.float64)
+    mos_kp = np.tile([NMOS["kp"], PMOS["kp"]], n_inv).astype(np.float64)
+    mos_w = np.tile([NMOS["w"], PMOS["w"]], n_inv).astype(np.float64)
+    mos_l = np.tile([NMOS["l"], PMOS["l"]], n_inv).astype(np.float64)
+    z32, z64 = np.zeros(0, np.int32), np.zeros(0, np.float64)
+    a = {"res_nodes": res_nodes.ravel(), "res_r": res_r, "cap_nodes": cap_nodes.ravel(), "cap_c": cap_c,
+         "ind_nodes": z32, "ind_l": z64, "vsrc_nodes": vsrc_nodes.ravel(), "vsrc_wave": vsrc_wave,
+         "isrc_nodes": z32, "isrc_wave": z32, "vccs_nodes": z32, "vccs_gain": z64, "vcvs_nodes": z32,
+         "vcvs_gain": z64, "mos_nodes": mos_nodes.ravel(), "mos_type": mos_type, "mos_vth": mos_vth,
+         "mos_kp": mos_kp, "mos_w": mos_w, "mos_l": mos_l, "wave_kind": wave_kind, "wave_off": wave_off,
+         "wave_len": wave_len, "wave_dc": wave_dc, "wave_data": wave_data}
+    node_names = [f"m{i}_{j}" for i in range(nx) for j in range(ny)] + ["vdd"] + \
+                 [f"in{k}" for k in range(n_inv)]
+    unknown_names = node_names + ["VDD.I"] + [f"VIN{k}.I" for k in range(n_inv)]
+    names = {nm_: i for i, nm_ in enumerate(unknown_names)}
+    device_names = {"VDD": ("vsrc", 0)}
+    assert nt == 2 * n_inv and len(unknown_names) == n_nodes + 1 + n_inv
+    return FlatCircuit(n_nodes, a, 1e-12, node_names, unknown_names, device_names, names)

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/cedarhip.h"
+#include "big.h"
 #include "compact.h"
 #include "kernels.h"
 #include "plan.h"
@@ -45,6 +46,8 @@ struct Circuit {
     std::string compact_why;
     CPlan hcp{};                // device pointers, host copy
     void* cbuf = nullptr;
+    // large circuits: the supernodal path (big.h) instead of all of the above
+    BigCircuit* big = nullptr;
 };
 
 constexpr size_t kLdsMaxBytes = 64 * 1024;   // dynamic LDS of one workgroup
@@ -344,6 +347,7 @@ int cs_destroy(cs_ctx* ctx) {
             if (c->wbuf) (void)hipFree(c->wbuf);
             if (c->h_norm) (void)hipHostFree(c->h_norm);
             if (c->cbuf) (void)hipFree(c->cbuf);
+            if (c->big) big_destroy(c->big);
         }
     for (hipStream_t st : ctx->side) (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(ctx->stream);
@@ -386,15 +390,57 @@ int cs_analyze(const cs_netlist* nl, cs_circuit_info* info, int32_t* lu_rowptr,
 
 // ---- circuits -----------------------------------------------------------------------
 int cs_circuit_load(cs_ctx* ctx, const cs_netlist* nl, int32_t* cid) {
+    return cs_circuit_load_solver(ctx, nl, CS_SOLVER_AUTO, cid);
+}
+
+int cs_circuit_load_solver(cs_ctx* ctx, const cs_netlist* nl, int32_t solver, int32_t* cid) {
     if (!ctx || !nl || !cid) return fail(ctx, CS_E_INVALID, "null argument");
+    if (solver < CS_SOLVER_AUTO || solver > CS_SOLVER_SUPERNODAL) return fail(ctx, CS_E_INVALID, "unknown solver");
     (void)hipSetDevice(ctx->device);
     auto C = std::make_unique<Circuit>();
     std::string err;
-    int rc = build_plan(nl, C->plan, err);
-    if (rc != CS_OK) return fail(ctx, rc, err);
-    if ((rc = upload_plan(ctx, *C)) != CS_OK) return rc;
+    const int64_t n = (int64_t)nl->n_nodes + nl->n_vsrc + nl->n_ind + nl->n_vcvs;
+    const bool big = solver == CS_SOLVER_SUPERNODAL || (solver == CS_SOLVER_AUTO && n >= CS_SUPERNODAL_MIN_N);
+    int rc;
+    if (big) {
+        rc = big_create(nl, ctx->stream, &C->big, err);
+        if (rc != CS_OK) return fail(ctx, rc, err);
+        C->plan = big_plan(C->big).base;
+    } else {
+        rc = build_plan(nl, C->plan, err);
+        if (rc != CS_OK) return fail(ctx, rc, err);
+        if ((rc = upload_plan(ctx, *C)) != CS_OK) return rc;
+    }
     ctx->circuits.push_back(std::move(C));
     *cid = (int32_t)ctx->circuits.size() - 1;
+    return CS_OK;
+}
+
+int cs_big_analyze(const cs_netlist* nl, cs_big_info* info, int32_t* rownat, int32_t* colnat) {
+    if (!nl || !info) return CS_E_INVALID;
+    BigPlan B;
+    std::string err;
+    int rc = build_bigplan(nl, B, err);
+    if (rc != CS_OK) return rc;
+    big_info(B, info);
+    if (rownat) std::memcpy(rownat, B.rownat.data(), sizeof(int32_t) * B.n);
+    if (colnat) std::memcpy(colnat, B.colnat.data(), sizeof(int32_t) * B.n);
+    return CS_OK;
+}
+
+int cs_big_info_get(cs_ctx* ctx, int32_t cid, cs_big_info* info) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || !info) return fail(ctx, CS_E_INVALID, "bad argument");
+    if (!C->big) return fail(ctx, CS_E_INVALID, "not a supernodal circuit");
+    big_info(big_plan(C->big), info);
+    return CS_OK;
+}
+
+int cs_big_times(cs_ctx* ctx, int32_t cid, double* factor_ms, double* solve_ms) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id");
+    if (!C->big) return fail(ctx, CS_E_INVALID, "not a supernodal circuit");
+    big_last_times(C->big, factor_ms, solve_ms);
     return CS_OK;
 }
 
@@ -439,6 +485,11 @@ int cs_circuit_set_probes(cs_ctx* ctx, int32_t cid, int32_t n_probe, const int32
     }
     set_probes(C->plan, v);
     (void)hipSetDevice(ctx->device);
+    if (C->big) {
+        std::string err;
+        int rc = big_set_probes(C->big, v, err);
+        return rc ? fail(ctx, rc, err) : CS_OK;
+    }
     return upload_plan(ctx, *C);
 }
 
@@ -448,6 +499,11 @@ int cs_residual(cs_ctx* ctx, int32_t cid, double* out, const double* du, const d
     Circuit* C = get_circuit(ctx, cid);
     if (!C || !out || !u || !p) return fail(ctx, CS_E_INVALID, "bad argument");
     (void)hipSetDevice(ctx->device);
+    if (C->big) {
+        std::string err;
+        int rc = big_residual(C->big, ctx->stream, out, du, u, p, t, mode, err);
+        return rc ? fail(ctx, rc, err) : CS_OK;
+    }
     int rc = ensure_workspace(ctx, *C);
     if (rc) return rc;
     const Plan& P = C->plan;
@@ -481,6 +537,11 @@ int cs_jacobian(cs_ctx* ctx, int32_t cid, double* vals, const double* du, const 
     Circuit* C = get_circuit(ctx, cid);
     if (!C || !vals || !u || !p) return fail(ctx, CS_E_INVALID, "bad argument");
     (void)hipSetDevice(ctx->device);
+    if (C->big) {
+        std::string err;
+        int rc = big_jacobian(C->big, ctx->stream, vals, du, u, p, gamma, t, mode, err);
+        return rc ? fail(ctx, rc, err) : CS_OK;
+    }
     int rc = ensure_workspace(ctx, *C);
     if (rc) return rc;
     const Plan& P = C->plan;
@@ -507,6 +568,11 @@ int cs_lu_refactor(cs_ctx* ctx, int32_t cid, const double* vals) {
     Circuit* C = get_circuit(ctx, cid);
     if (!C || !vals) return fail(ctx, CS_E_INVALID, "bad argument");
     (void)hipSetDevice(ctx->device);
+    if (C->big) {
+        std::string err;
+        int rc = big_lu_refactor(C->big, ctx->stream, vals, err);
+        return rc ? fail(ctx, rc, err) : CS_OK;
+    }
     int rc = ensure_workspace(ctx, *C);
     if (rc) return rc;
     const Plan& P = C->plan;
@@ -526,6 +592,13 @@ int cs_lu_refactor(cs_ctx* ctx, int32_t cid, const double* vals) {
 static int lu_solve_common(cs_ctx* ctx, int32_t cid, double* x, bool transpose) {
     Circuit* C = get_circuit(ctx, cid);
     if (!C || !x) return fail(ctx, CS_E_INVALID, "bad argument");
+    if (C->big) {
+        if (transpose) return fail(ctx, CS_E_INVALID, "transposed solves are not available for supernodal circuits");
+        (void)hipSetDevice(ctx->device);
+        std::string err;
+        int rc = big_lu_solve(C->big, ctx->stream, x, err);
+        return rc ? fail(ctx, rc, err) : CS_OK;
+    }
     if (!C->wbuf) return fail(ctx, CS_E_INVALID, "cs_lu_refactor must be called first");
     (void)hipSetDevice(ctx->device);
     const Plan& P = C->plan;
@@ -578,6 +651,12 @@ int cs_dcop(cs_ctx* ctx, int32_t cid, const double* p, const cs_tran_opts* o, do
     int rc = check_params(ctx, C->plan, p);
     if (rc) return rc;
     (void)hipSetDevice(ctx->device);
+    if (C->big) {
+        std::string err;
+        rc = big_dcop(C->big, ctx->stream, p, o, u_out, st, err);
+        if (rc) ctx->err = err;
+        return rc;
+    }
     if ((rc = ensure_workspace(ctx, *C)) || (rc = ensure_graph(ctx, *C))) return rc;
     const Plan& P = C->plan;
     HIPCHK(ctx, hipMemcpyAsync(C->w.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, ctx->stream));
@@ -601,6 +680,13 @@ int cs_tran(cs_ctx* ctx, int32_t cid, const double* p, const double* u0, const c
     if (!C || !o) return fail(ctx, CS_E_INVALID, "bad argument");
     int rc = check_params(ctx, C->plan, p);
     if (rc) return rc;
+    if (C->big) {
+        (void)hipSetDevice(ctx->device);
+        std::string err;
+        rc = big_tran(C->big, ctx->stream, p, u0, o, wave, u_final, st_out, err);
+        if (rc) ctx->err = err;
+        return rc;
+    }
     const Plan& P = C->plan;
     const bool uic = o->init == CS_INIT_UIC;
     if (uic && !u0) return fail(ctx, CS_E_INVALID, "UIC initialisation needs u0");
@@ -916,6 +1002,7 @@ int cs_batch_create(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const 
     for (int32_t i = 0; i < n_inst; i++) {
         Circuit* C = get_circuit(ctx, inst_cid[i]);
         if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id in batch");
+        if (C->big) return fail(ctx, CS_E_INVALID, "supernodal circuits run on cs_tran, not in batches");
         const Plan& P = C->plan;
         if (o->init == CS_INIT_UIC && !P.uic_ok)
             return fail(ctx, CS_E_INVALID, "UIC: a charge-carrying row could not be kept on its own pivot");
@@ -1257,6 +1344,7 @@ int cs_adjoint_batch(cs_ctx* ctx, int32_t n_inst, const int32_t* inst_cid, const
     for (int32_t i = 0; i < n_inst; i++) {
         Circuit* C = get_circuit(ctx, inst_cid[i]);
         if (!C) return fail(ctx, CS_E_INVALID, "bad circuit id");
+        if (C->big) return fail(ctx, CS_E_INVALID, "supernodal circuits have no batched adjoint");
         const Plan& P = C->plan;
         if (!P.uic_ok) return fail(ctx, CS_E_INVALID, "UIC not supported by this circuit");
         if (ao->vsrc < 0 || 2 * ao->vsrc + 1 >= (int32_t)P.vsrc_nodes.size())

@@ -1,0 +1,1112 @@
+// big.hip — large-circuit path: device-wide evaluation and assembly, the
+// supernodal multifrontal LU of bigplan.h, and the transient / DC-op drivers
+// with the step history kept on the device.
+//
+// One Newton iteration (captured once in a hipGraph and replayed):
+//   eval      one lane per MOSFET / source (grid-stride)
+//   assemble  one lane per natural row: Jacobian values on the natural
+//             pattern (fixed-order contributor gathers) and -residual in
+//             pivot order; per-block max |residual|
+//   factor    for each assembly-tree level, leaves first:
+//               gather  front entries = sum of the children's contribution
+//                       blocks (inverse relative maps; every entry written
+//                       once, children in a fixed order: no atomics)
+//               orig    + the level's Jacobian entries (distinct positions)
+//               small   fronts with m <= kSmallM: one workgroup each, the
+//                       whole front in LDS, right-looking partial LU
+//               large   blocked right-looking partial LU, panel width kNb:
+//                       diagonal block (one workgroup), row / column
+//                       triangular solves, trailing update by 64 x 64 tiles
+//   solve     forward by levels (children's update vectors gathered in a
+//             fixed order), backward from the root
+//   update    u += dx, WRMS / max norms (fixed-order block reductions)
+// Everything is bit-reproducible run to run.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "big.h"
+#include "kernels.h"
+#include "rules.h"
+
+namespace cedar {
+
+namespace {
+
+constexpr int kBT = 256;      // threads of the device-wide kernels
+constexpr int kSmallM = 96;   // fronts up to this size factor in LDS
+constexpr int kNb = 32;       // panel width of the blocked front LU
+constexpr int kTile = 64;     // trailing-update tile
+constexpr int kGatherChunk = 8192;
+constexpr int kRowBlocks = 1024;
+
+struct BigDev {
+    int32_t n, nnz, n_mos, n_src, n_lin, n_stamp, n_params;
+    int32_t p_mos_vth, p_mos_kp, p_mos_w, p_mos_l, p_gmin;
+    const int32_t *rowptr, *col, *lin_kind, *lin_idx, *g_ptr, *g_ref, *c_ptr, *c_ref;
+    const int32_t *jnl_ptr, *jnl_ref, *rnl_ptr, *rnl_ref, *is_diff, *prow, *pcol;
+    const int4 *mos, *src;
+    const int32_t *sn_first, *sn_m, *sn_rptr, *sn_rows, *ch_ptr, *ch_sn, *rel_ptr, *rel, *inv, *olev_pos;
+    const int64_t *sn_off, *inv_ptr, *nat_dest;
+};
+
+__device__ __forceinline__ double sref_v(const double* buf, int32_t r) {
+    const double v = buf[r >> 1];
+    return (r & 1) ? -v : v;
+}
+
+template <class Op>
+__device__ __forceinline__ double blk_reduce(double v, double* red, Op op) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = op(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+struct Sum {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct Max {   // NaN-propagating
+    __device__ double operator()(double a, double b) const { return (a != a || a > b) ? a : b; }
+};
+
+// ---- evaluation and assembly ----------------------------------------------------------
+__global__ __launch_bounds__(kBT) void k_lin_a(const BigDev* P, const double* p, double* linbuf) {
+    for (int s = blockIdx.x * kBT + threadIdx.x; s < P->n_lin; s += gridDim.x * kBT) {
+        const int kd = P->lin_kind[s];
+        const double pv = kd == 0 ? 1.0 : p[P->lin_idx[s]];
+        linbuf[s] = kd == 0 ? 1.0 : kd == 1 ? 1.0 / pv : kd == 2 ? pv : -pv;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_lin_b(const BigDev* P, const double* linbuf, double* Glin,
+                                               double* Clin) {
+    for (int e = blockIdx.x * kBT + threadIdx.x; e < P->nnz; e += gridDim.x * kBT) {
+        double g = 0.0, c = 0.0;
+        for (int q = P->g_ptr[e]; q < P->g_ptr[e + 1]; q++) g += sref_v(linbuf, P->g_ref[q]);
+        for (int q = P->c_ptr[e]; q < P->c_ptr[e + 1]; q++) c += sref_v(linbuf, P->c_ref[q]);
+        Glin[e] = g;
+        Clin[e] = c;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_eval(const BigDev* P, const double* p, const double* u,
+                                              const StepDev* spp, double* st) {
+    const StepDev sp = *spp;
+    const double gmin = p[P->p_gmin];
+    const int total = P->n_mos + P->n_src;
+    for (int q = blockIdx.x * kBT + threadIdx.x; q < total; q += gridDim.x * kBT) {
+        if (q < P->n_mos) {
+            const int4 dv = P->mos[q];
+            const double vd = dv.x >= 0 ? u[dv.x] : 0.0;
+            const double vg = dv.y >= 0 ? u[dv.y] : 0.0;
+            const double vs = dv.z >= 0 ? u[dv.z] : 0.0;
+            double id, dd, dg, ds;
+            sqlaw_eval(dv.w, p[P->p_mos_vth + q], p[P->p_mos_kp + q], p[P->p_mos_w + q],
+                       p[P->p_mos_l + q], gmin, vd, vg, vs, id, dd, dg, ds);
+            st[4 * q] = id;
+            st[4 * q + 1] = dd;
+            st[4 * q + 2] = dg;
+            st[4 * q + 3] = ds;
+        } else {
+            const int s = q - P->n_mos;
+            const int4 d = P->src[s];
+            st[4 * P->n_mos + s] = source_value(d.x, p + d.y, d.z, p[d.w], sp.t, sp.mode);
+        }
+    }
+}
+
+// Jacobian values on the natural pattern and x[prow[r]] = -residual(r)
+// (x may be null).  a0 and the charge term qx apply in transient steps;
+// pinned rows (UIC) become u_r = pin_r.
+__global__ __launch_bounds__(kBT) void k_assemble(const BigDev* P, const double* Glin, const double* Clin,
+                                                  const StepDev* spp, const double* u, const double* qx,
+                                                  const double* st, const double* pin, double* Aval,
+                                                  double* x, double* part) {
+    __shared__ double red[kBT];
+    const StepDev sp = *spp;
+    const double a0 = sp.kind == STEP_TRAN ? sp.a0 : 0.0;
+    const bool use_q = sp.kind == STEP_TRAN && qx;
+    const bool use_pin = sp.kind == STEP_UIC && pin;
+    double lmax = 0.0;
+    for (int r = blockIdx.x * kBT + threadIdx.x; r < P->n; r += gridDim.x * kBT) {
+        const int e0 = P->rowptr[r], e1 = P->rowptr[r + 1];
+        double res;
+        if (use_pin && P->is_diff[r]) {
+            for (int e = e0; e < e1; e++) Aval[e] = P->col[e] == r ? 1.0 : 0.0;
+            res = u[r] - pin[r];
+        } else {
+            double acc = 0.0;
+            for (int e = e0; e < e1; e++) {
+                const double jl = Glin[e] + a0 * Clin[e];
+                acc += jl * u[P->col[e]];
+                double v = jl;
+                for (int q = P->jnl_ptr[e]; q < P->jnl_ptr[e + 1]; q++) v += sref_v(st, P->jnl_ref[q]);
+                Aval[e] = v;
+            }
+            if (use_q) acc += qx[r];
+            for (int q = P->rnl_ptr[r]; q < P->rnl_ptr[r + 1]; q++) acc += sref_v(st, P->rnl_ref[q]);
+            res = acc;
+        }
+        if (x) x[P->prow[r]] = -res;
+        const double a = fabs(res);
+        if (a != a || a > lmax) lmax = a;
+    }
+    const double m = blk_reduce(lmax, red, Max());
+    if (threadIdx.x == 0 && part) part[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kBT) void k_res_reduce(const double* part, int nb, double* norm) {
+    __shared__ double red[kBT];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBT) {
+        const double v = part[b];
+        if (v != v || v > m) m = v;
+    }
+    m = blk_reduce(m, red, Max());
+    if (threadIdx.x == 0) norm[NRM_RES] = m;
+}
+
+// u += dx (DC / UIC: skipped once the residual met the tolerance); block
+// partial sums of the WRMS terms and maxima of |dx|.
+__global__ __launch_bounds__(kBT) void k_update(const BigDev* P, const StepDev* spp, const double* x,
+                                                double* u, double* norm, double* part) {
+    __shared__ double red[kBT];
+    const StepDev sp = *spp;
+    if (sp.kind != STEP_TRAN && norm[NRM_RES] <= sp.tol) return;
+    double ss = 0.0, mx = 0.0;
+    for (int i = blockIdx.x * kBT + threadIdx.x; i < P->n; i += gridDim.x * kBT) {
+        const double d = x[P->pcol[i]];
+        const double ui = u[i] + d;
+        u[i] = ui;
+        const double r = d / (sp.rtol * fabs(ui) + sp.atol);
+        ss += r * r;
+        const double a = fabs(d);
+        if (a != a || a > mx) mx = a;
+    }
+    const double s = blk_reduce(ss, red, Sum());
+    const double m = blk_reduce(mx, red, Max());
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s;
+        part[2 * blockIdx.x + 1] = m;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_upd_reduce(const BigDev* P, const StepDev* spp, const double* part,
+                                                    int nb, double* norm) {
+    __shared__ double red[kBT];
+    const StepDev sp = *spp;
+    if (sp.kind != STEP_TRAN && norm[NRM_RES] <= sp.tol) {
+        if (threadIdx.x == 0) norm[NRM_DONE] = 1.0;
+        return;
+    }
+    double s = 0.0, m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBT) {
+        s += part[2 * b];
+        const double v = part[2 * b + 1];
+        if (v != v || v > m) m = v;
+    }
+    s = blk_reduce(s, red, Sum());
+    m = blk_reduce(m, red, Max());
+    if (threadIdx.x == 0) {
+        norm[NRM_WRMS] = sqrt(s / P->n);
+        norm[NRM_MAXD] = m;
+        norm[NRM_DONE] = 0.0;
+    }
+}
+
+// ---- multifrontal factorization ----------------------------------------------------------
+// Front s: m x m column-major at pool + sn_off[s]; k = pivots; CB = trailing r x r.
+__device__ __forceinline__ int sn_k(const BigDev* P, int s) { return P->sn_first[s + 1] - P->sn_first[s]; }
+
+// tile = (front, first column, end column): entries = sum of children's CB entries
+__global__ __launch_bounds__(kBT) void k_mf_gather(const BigDev* P, double* pool, const int4* tiles) {
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, m = P->sn_m[s];
+    double* F = pool + P->sn_off[s];
+    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
+    const int cnt = (t.z - t.y) * m;
+    for (int q = threadIdx.x; q < cnt; q += kBT) {
+        const int j = t.y + q / m, i = q % m;
+        double v = 0.0;
+        for (int cq = c0; cq < c1; cq++) {
+            const int c = P->ch_sn[cq];
+            const int* iv = P->inv + P->inv_ptr[c];
+            const int a = iv[i], b = iv[j];
+            if (a >= 0 && b >= 0) {
+                const int mc = P->sn_m[c], kc = sn_k(P, c);
+                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
+            }
+        }
+        F[(int64_t)j * m + i] = v;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* Aval, double* pool, int b,
+                                                 int e) {
+    for (int q = b + blockIdx.x * kBT + threadIdx.x; q < e; q += gridDim.x * kBT) {
+        const int pos = P->olev_pos[q];
+        pool[P->nat_dest[pos]] += Aval[pos];
+    }
+}
+
+// Whole front in LDS; right-looking partial LU over its k pivots.
+__global__ __launch_bounds__(kBT) void k_mf_small(const BigDev* P, double* pool, double* dinv,
+                                                  const int32_t* fronts) {
+    extern __shared__ double lds_f[];
+    double* F = lds_f;
+    const int s = fronts[blockIdx.x];
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    double* G = pool + P->sn_off[s];
+    for (int q = threadIdx.x; q < m * m; q += kBT) F[q] = G[q];
+    __syncthreads();
+    for (int p = 0; p < k; p++) {
+        const double ip = 1.0 / F[p * m + p];
+        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) F[p * m + i] *= ip;
+        if (threadIdx.x == 0) dinv[f0 + p] = ip;
+        __syncthreads();
+        const int w = m - p - 1;
+        for (int q = threadIdx.x; q < w * w; q += kBT) {
+            const int j = p + 1 + q / w, i = p + 1 + q % w;
+            F[j * m + i] -= F[p * m + i] * F[j * m + p];
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < m * m; q += kBT) G[q] = F[q];
+}
+
+// item = (front, p0): LU of the diagonal block [p0, p0+nb) x [p0, p0+nb)
+__global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, double* dinv, const int2* items) {
+    __shared__ double D[kNb * kNb];
+    const int2 it = items[blockIdx.x];
+    const int s = it.x, p0 = it.y;
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+        const int j = q / nb, i = q % nb;
+        D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
+    }
+    __syncthreads();
+    for (int p = 0; p < nb; p++) {
+        const double ip = 1.0 / D[p * kNb + p];
+        for (int i = p + 1 + threadIdx.x; i < nb; i += kBT) D[p * kNb + i] *= ip;
+        if (threadIdx.x == 0) dinv[f0 + p0 + p] = ip;
+        __syncthreads();
+        const int w = nb - p - 1;
+        for (int q = threadIdx.x; q < w * w; q += kBT) {
+            const int j = p + 1 + q / w, i = p + 1 + q % w;
+            D[j * kNb + i] -= D[p * kNb + i] * D[j * kNb + p];
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+        const int j = q / nb, i = q % nb;
+        G[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
+    }
+}
+
+// item = (front, p0, kind, first): kind 0 solves rows [first, first+256) of
+// L21 (x U11 = a), kind 1 columns of U12 (L11 y = a, unit diagonal)
+__global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, const double* dinv,
+                                                 const int4* items) {
+    __shared__ double D[kNb * kNb];
+    const int4 it = items[blockIdx.x];
+    const int s = it.x, p0 = it.y;
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+        const int j = q / nb, i = q % nb;
+        D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
+    }
+    __syncthreads();
+    const int r = it.w + threadIdx.x;
+    if (r >= m) return;
+    double x[kNb];
+    if (it.z == 0) {
+        for (int c = 0; c < nb; c++) x[c] = G[(int64_t)(p0 + c) * m + r];
+        for (int c = 0; c < nb; c++) {
+            double v = x[c];
+            for (int c2 = 0; c2 < c; c2++) v -= x[c2] * D[c * kNb + c2];
+            x[c] = v * dinv[f0 + p0 + c];
+        }
+        for (int c = 0; c < nb; c++) G[(int64_t)(p0 + c) * m + r] = x[c];
+    } else {
+        double* col = G + (int64_t)r * m + p0;
+        for (int c = 0; c < nb; c++) x[c] = col[c];
+        for (int c = 0; c < nb; c++) {
+            double v = x[c];
+            for (int c2 = 0; c2 < c; c2++) v -= D[c2 * kNb + c] * x[c2];
+            x[c] = v;
+        }
+        for (int c = 0; c < nb; c++) col[c] = x[c];
+    }
+}
+
+// tile = (front, p0, i0, j0): F[i0:i0+64, j0:j0+64] -= L[.., p0:p0+nb] U[p0:p0+nb, ..]
+__global__ __launch_bounds__(kBT) void k_mf_gemm(const BigDev* P, double* pool, const int4* tiles) {
+    __shared__ double A[kNb][kTile + 1];   // L panel, transposed: A[c][i]
+    __shared__ double Bt[kNb][kTile + 1];  // U panel: Bt[c][j]
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
+    const int m = P->sn_m[s], k = sn_k(P, s);
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
+        const int c = q / kTile, i = q % kTile;
+        const bool ok = c < nb;
+        A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
+        Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int ti = (threadIdx.x % 16) * 4, tj = (threadIdx.x / 16) * 4;
+    double acc[4][4] = {};
+    for (int c = 0; c < nb; c++) {
+        double a[4], b[4];
+        for (int q = 0; q < 4; q++) { a[q] = A[c][ti + q]; b[q] = Bt[c][tj + q]; }
+        for (int y = 0; y < 4; y++)
+            for (int x = 0; x < 4; x++) acc[y][x] += a[x] * b[y];
+    }
+    for (int y = 0; y < 4; y++) {
+        const int j = j0 + tj + y;
+        if (j >= m) continue;
+        for (int x = 0; x < 4; x++) {
+            const int i = i0 + ti + x;
+            if (i < m) G[(int64_t)j * m + i] -= acc[y][x];
+        }
+    }
+}
+
+// ---- multifrontal solves ----------------------------------------------------------------
+// forward: y = L^-1 b by levels; update vectors of the fronts in uvec
+__global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
+                                                const int32_t* fronts) {
+    extern __shared__ double v[];
+    const int s = fronts[blockIdx.x];
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const double* G = pool + P->sn_off[s];
+    for (int a = threadIdx.x; a < m; a += kBT) v[a] = a < k ? x[f0 + a] : 0.0;
+    __syncthreads();
+    for (int cq = P->ch_ptr[s]; cq < P->ch_ptr[s + 1]; cq++) {
+        const int c = P->ch_sn[cq];
+        const int rc = P->sn_rptr[c + 1] - P->sn_rptr[c];
+        for (int a = threadIdx.x; a < rc; a += kBT) v[P->rel[P->rel_ptr[c] + a]] += uvec[P->rel_ptr[c] + a];
+        __syncthreads();
+    }
+    for (int p = 0; p < k; p++) {
+        const double y = v[p];
+        const double* col = G + (int64_t)p * m;
+        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) v[i] -= col[i] * y;
+        __syncthreads();
+    }
+    for (int a = threadIdx.x; a < m; a += kBT) {
+        if (a < k) x[f0 + a] = v[a];
+        else uvec[P->rel_ptr[s] + (a - k)] = v[a];
+    }
+}
+
+// backward: x = U^-1 y from the root down
+__global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* pool, const double* dinv,
+                                                double* x, const int32_t* fronts) {
+    extern __shared__ double w[];
+    const int s = fronts[blockIdx.x];
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int r = m - k;
+    const double* G = pool + P->sn_off[s];
+    const int32_t* rows = P->sn_rows + P->sn_rptr[s];
+    double* xu = w + k;
+    for (int a = threadIdx.x; a < r; a += kBT) xu[a] = x[rows[a]];
+    __syncthreads();
+    for (int p = threadIdx.x; p < k; p += kBT) {
+        double acc = x[f0 + p];
+        for (int a = 0; a < r; a++) acc -= G[(int64_t)(k + a) * m + p] * xu[a];
+        w[p] = acc;
+    }
+    __syncthreads();
+    for (int p = k - 1; p >= 0; p--) {
+        const double xp = w[p] * dinv[f0 + p];
+        __syncthreads();
+        if (threadIdx.x == 0) w[p] = xp;
+        const double* col = G + (int64_t)p * m;
+        for (int i = threadIdx.x; i < p; i += kBT) w[i] -= col[i] * xp;
+        __syncthreads();
+    }
+    for (int p = threadIdx.x; p < k; p += kBT) x[f0 + p] = w[p];
+}
+
+// ---- step controller ----------------------------------------------------------------------
+__global__ __launch_bounds__(kBT) void k_predict(int n, int np, int k, double w0, double w1, double w2,
+                                                 double a1, double a2, const double* U0, const double* U1,
+                                                 const double* U2, const double* Q0, const double* Q1,
+                                                 double* up, double* u, double* qx) {
+    for (int i = blockIdx.x * kBT + threadIdx.x; i < n; i += gridDim.x * kBT) {
+        double v = U0[i];
+        if (np == 2) v = w0 * U0[i] + w1 * U1[i];
+        else if (np == 3) v = w0 * U0[i] + w1 * U1[i] + w2 * U2[i];
+        up[i] = v;
+        u[i] = v;
+        qx[i] = a1 * Q0[i] + (k == 2 ? a2 * Q1[i] : 0.0);
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_charges(const BigDev* P, const double* Clin, const double* v,
+                                                 double* q) {
+    for (int r = blockIdx.x * kBT + threadIdx.x; r < P->n; r += gridDim.x * kBT) {
+        double acc = 0.0;
+        for (int e = P->rowptr[r]; e < P->rowptr[r + 1]; e++) acc += Clin[e] * v[P->col[e]];
+        q[r] = acc;
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_lte(int n, double c, double rtol, double atol, const double* u,
+                                             const double* up, const double* U0, double* part) {
+    __shared__ double red[kBT];
+    double ss = 0.0;
+    for (int i = blockIdx.x * kBT + threadIdx.x; i < n; i += gridDim.x * kBT) {
+        const double est = c * (u[i] - up[i]);
+        const double m = fmax(fabs(u[i]), fabs(U0[i]));
+        const double r = est / (rtol * m + atol);
+        ss += r * r;
+    }
+    const double s = blk_reduce(ss, red, Sum());
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBT) void k_sum_reduce(const double* part, int nb, double* out) {
+    __shared__ double red[kBT];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBT) s += part[b];
+    s = blk_reduce(s, red, Sum());
+    if (threadIdx.x == 0) *out = s;
+}
+
+// output samples: wave[(kout) * n_probe + slot] = l0 U0 + l1 U1 + l2 U2 at the probes
+__global__ __launch_bounds__(kBT) void k_wave(int n_probe, const int32_t* probe, const double* U0,
+                                              const double* U1, const double* U2, double l0, double l1,
+                                              double l2, double* row) {
+    for (int s = blockIdx.x * kBT + threadIdx.x; s < n_probe; s += gridDim.x * kBT) {
+        const int i = probe[s];
+        row[s] = l0 * U0[i] + l1 * U1[i] + l2 * U2[i];
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_perm_in(int n, const int32_t* prow, const double* b, double* x) {
+    for (int r = blockIdx.x * kBT + threadIdx.x; r < n; r += gridDim.x * kBT) x[prow[r]] = b[r];
+}
+__global__ __launch_bounds__(kBT) void k_perm_out(int n, const int32_t* pcol, const double* x, double* out) {
+    for (int c = blockIdx.x * kBT + threadIdx.x; c < n; c += gridDim.x * kBT) out[c] = x[pcol[c]];
+}
+
+int grid_for(int64_t work) {
+    const int64_t b = (work + kBT - 1) / kBT;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRowBlocks));
+}
+
+}  // namespace
+
+// ============================================================================================
+struct BigCircuit {
+    BigPlan B;
+    BigDev hd{};
+    BigDev* d_dev = nullptr;
+    void* dbuf = nullptr;   // plan arrays
+    void* wbuf = nullptr;   // workspace
+    double *p = nullptr, *linbuf = nullptr, *Glin = nullptr, *Clin = nullptr, *Aval = nullptr, *pool = nullptr;
+    double *dinv = nullptr, *u = nullptr, *x = nullptr, *qx = nullptr, *u0 = nullptr, *st = nullptr;
+    double *uvec = nullptr, *part = nullptr, *norm = nullptr, *up = nullptr, *qn = nullptr;
+    double* H[3] = {nullptr, nullptr, nullptr};
+    double* Q[2] = {nullptr, nullptr};
+    double* scal = nullptr;
+    StepDev* sp = nullptr;
+    double* h_norm = nullptr;   // pinned
+    int32_t* d_probe = nullptr;
+    // launch tables
+    struct Panel {
+        int diag_b, diag_n, trsm_b, trsm_n, gemm_b, gemm_n;
+    };
+    struct Level {
+        int tile_b, tile_n, small_b, small_n, fr_b, fr_n, orig_b, orig_e;
+        size_t small_lds, solve_lds;
+        std::vector<Panel> panels;
+    };
+    std::vector<Level> levels;
+    int4* d_tiles = nullptr;
+    int32_t* d_fronts = nullptr;
+    int2* d_diag = nullptr;
+    int4* d_trsm = nullptr;
+    int4* d_gemm = nullptr;
+    void* tbuf = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double factor_ms = 0.0, solve_ms = 0.0;
+    bool lin_ready = false;
+};
+
+namespace {
+
+template <class T>
+size_t pack_add(std::vector<char>& bytes, const std::vector<T>& v) {
+    size_t off = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+#define BCHK(expr)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+            return e_ == hipErrorOutOfMemory ? CS_E_OOM : CS_E_HIP;                 \
+        }                                                                           \
+    } while (0)
+
+int upload(BigCircuit& C, std::string& err) {
+    const BigPlan& B = C.B;
+    const Plan& P = B.base;
+    std::vector<char> by;
+    std::vector<int4> mos4, src4;
+    for (int32_t q = 0; q < P.n_mos; q++)
+        mos4.push_back(make_int4(P.mos[4 * q], P.mos[4 * q + 1], P.mos[4 * q + 2], P.mos[4 * q + 3]));
+    for (auto& s : P.src) src4.push_back(make_int4(s.kind, s.off, s.len, s.dc));
+    size_t o_rp = pack_add(by, P.nat_rowptr), o_col = pack_add(by, P.nat_col), o_lk = pack_add(by, P.lin_kind),
+           o_li = pack_add(by, P.lin_idx), o_gp = pack_add(by, P.g_ptr), o_gr = pack_add(by, P.g_ref),
+           o_cp = pack_add(by, P.c_ptr), o_cr = pack_add(by, P.c_ref), o_jp = pack_add(by, P.jnl_ptr),
+           o_jr = pack_add(by, P.jnl_ref), o_rnp = pack_add(by, P.rnl_ptr), o_rnr = pack_add(by, P.rnl_ref),
+           o_isd = pack_add(by, P.is_diff), o_prow = pack_add(by, B.prow), o_pcol = pack_add(by, B.pcol),
+           o_mos = pack_add(by, mos4), o_src = pack_add(by, src4), o_sf = pack_add(by, B.sn_first),
+           o_sm = pack_add(by, B.sn_m), o_srp = pack_add(by, B.sn_rptr), o_srow = pack_add(by, B.sn_rows),
+           o_chp = pack_add(by, B.ch_ptr), o_chs = pack_add(by, B.ch_sn), o_relp = pack_add(by, B.rel_ptr),
+           o_rel = pack_add(by, B.rel), o_inv = pack_add(by, B.inv), o_olp = pack_add(by, B.olev_pos),
+           o_soff = pack_add(by, B.sn_off), o_invp = pack_add(by, B.inv_ptr), o_nd = pack_add(by, B.nat_dest),
+           o_probe = pack_add(by, P.probe);
+    BCHK(hipMalloc(&C.dbuf, by.size()));
+    BCHK(hipMemcpy(C.dbuf, by.data(), by.size(), hipMemcpyHostToDevice));
+    char* b = (char*)C.dbuf;
+    auto I = [&](size_t o) { return (const int32_t*)(b + o); };
+    auto L = [&](size_t o) { return (const int64_t*)(b + o); };
+    BigDev& d = C.hd;
+    d.n = B.n; d.nnz = P.nnz_nat; d.n_mos = P.n_mos; d.n_src = P.n_src; d.n_lin = P.n_lin;
+    d.n_stamp = P.n_stamp; d.n_params = P.lay.n_params;
+    d.p_mos_vth = P.lay.mos_vth; d.p_mos_kp = P.lay.mos_kp; d.p_mos_w = P.lay.mos_w;
+    d.p_mos_l = P.lay.mos_l; d.p_gmin = P.lay.gmin;
+    d.rowptr = I(o_rp); d.col = I(o_col); d.lin_kind = I(o_lk); d.lin_idx = I(o_li);
+    d.g_ptr = I(o_gp); d.g_ref = I(o_gr); d.c_ptr = I(o_cp); d.c_ref = I(o_cr);
+    d.jnl_ptr = I(o_jp); d.jnl_ref = I(o_jr); d.rnl_ptr = I(o_rnp); d.rnl_ref = I(o_rnr);
+    d.is_diff = I(o_isd); d.prow = I(o_prow); d.pcol = I(o_pcol);
+    d.mos = (const int4*)(b + o_mos); d.src = (const int4*)(b + o_src);
+    d.sn_first = I(o_sf); d.sn_m = I(o_sm); d.sn_rptr = I(o_srp); d.sn_rows = I(o_srow);
+    d.ch_ptr = I(o_chp); d.ch_sn = I(o_chs); d.rel_ptr = I(o_relp); d.rel = I(o_rel); d.inv = I(o_inv);
+    d.olev_pos = I(o_olp); d.sn_off = L(o_soff); d.inv_ptr = L(o_invp); d.nat_dest = L(o_nd);
+    C.d_probe = (int32_t*)(b + o_probe);
+    BCHK(hipMalloc(&C.d_dev, sizeof(BigDev)));
+    BCHK(hipMemcpy(C.d_dev, &C.hd, sizeof(BigDev), hipMemcpyHostToDevice));
+
+    // launch tables
+    std::vector<int4> tiles, trsm, gemm;
+    std::vector<int32_t> fronts;
+    std::vector<int2> diag;
+    const int nlev = (int)B.lev_ptr.size() - 1;
+    C.levels.resize(nlev);
+    for (int l = 0; l < nlev; l++) {
+        auto& Lv = C.levels[l];
+        Lv.tile_b = (int)tiles.size();
+        int small_max = 0, m_max = 0;
+        std::vector<int32_t> small, large;
+        for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) {
+            const int s = B.lev_sn[q], m = B.sn_m[s];
+            const int step = std::max(1, kGatherChunk / std::max(m, 1));
+            for (int j = 0; j < m; j += step) tiles.push_back(make_int4(s, j, std::min(m, j + step), 0));
+            if (m <= kSmallM) { small.push_back(s); small_max = std::max(small_max, m); }
+            else large.push_back(s);
+            m_max = std::max(m_max, m);
+        }
+        Lv.tile_n = (int)tiles.size() - Lv.tile_b;
+        Lv.small_b = (int)fronts.size();
+        fronts.insert(fronts.end(), small.begin(), small.end());
+        Lv.small_n = (int)small.size();
+        Lv.small_lds = sizeof(double) * (size_t)small_max * small_max;
+        Lv.fr_b = (int)fronts.size();
+        for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) fronts.push_back(B.lev_sn[q]);
+        Lv.fr_n = (int)fronts.size() - Lv.fr_b;
+        Lv.solve_lds = sizeof(double) * (size_t)std::max(m_max, 1);
+        Lv.orig_b = B.olev_ptr[l];
+        Lv.orig_e = B.olev_ptr[l + 1];
+        int kmax = 0;
+        for (int s : large) kmax = std::max(kmax, B.sn_first[s + 1] - B.sn_first[s]);
+        for (int p0 = 0; p0 < kmax; p0 += kNb) {
+            BigCircuit::Panel pn{};
+            pn.diag_b = (int)diag.size();
+            pn.trsm_b = (int)trsm.size();
+            pn.gemm_b = (int)gemm.size();
+            for (int s : large) {
+                const int k = B.sn_first[s + 1] - B.sn_first[s], m = B.sn_m[s];
+                if (p0 >= k) continue;
+                const int p1 = std::min(p0 + kNb, k);
+                diag.push_back(make_int2(s, p0));
+                for (int r0 = p1; r0 < m; r0 += kBT) {
+                    trsm.push_back(make_int4(s, p0, 0, r0));
+                    trsm.push_back(make_int4(s, p0, 1, r0));
+                }
+                for (int j0 = p1; j0 < m; j0 += kTile)
+                    for (int i0 = p1; i0 < m; i0 += kTile) gemm.push_back(make_int4(s, p0, i0, j0));
+            }
+            pn.diag_n = (int)diag.size() - pn.diag_b;
+            pn.trsm_n = (int)trsm.size() - pn.trsm_b;
+            pn.gemm_n = (int)gemm.size() - pn.gemm_b;
+            Lv.panels.push_back(pn);
+        }
+    }
+    std::vector<char> tb;
+    size_t o_t = pack_add(tb, tiles), o_f = pack_add(tb, fronts), o_d = pack_add(tb, diag),
+           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm);
+    BCHK(hipMalloc(&C.tbuf, tb.size()));
+    BCHK(hipMemcpy(C.tbuf, tb.data(), tb.size(), hipMemcpyHostToDevice));
+    char* t = (char*)C.tbuf;
+    C.d_tiles = (int4*)(t + o_t);
+    C.d_fronts = (int32_t*)(t + o_f);
+    C.d_diag = (int2*)(t + o_d);
+    C.d_trsm = (int4*)(t + o_tr);
+    C.d_gemm = (int4*)(t + o_g);
+
+    // workspace
+    const size_t n = B.n, nnz = P.nnz_nat;
+    const size_t nd = P.lay.n_params + P.n_lin + 3 * nnz + (size_t)B.pool + 16 * n + P.n_stamp +
+                      B.rel_ptr[B.ns] + 2 * kRowBlocks + NRM_SLOTS + 8 + sizeof(StepDev) / 8 + 64;
+    BCHK(hipMalloc(&C.wbuf, nd * sizeof(double)));
+    BCHK(hipMemset(C.wbuf, 0, nd * sizeof(double)));
+    double* q = (double*)C.wbuf;
+    auto take = [&](size_t k) { double* r = q; q += (k + 1) & ~size_t(1); return r; };
+    C.p = take(P.lay.n_params);
+    C.linbuf = take(P.n_lin);
+    C.Glin = take(nnz);
+    C.Clin = take(nnz);
+    C.Aval = take(nnz);
+    C.pool = take(B.pool);
+    C.dinv = take(n);
+    C.u = take(n);
+    C.x = take(n);
+    C.qx = take(n);
+    C.u0 = take(n);
+    C.up = take(n);
+    C.qn = take(n);
+    for (auto& h : C.H) h = take(n);
+    for (auto& h : C.Q) h = take(n);
+    C.st = take(std::max<int32_t>(P.n_stamp, 1));
+    C.uvec = take(std::max<int32_t>(B.rel_ptr[B.ns], 1));
+    C.part = take(2 * kRowBlocks);
+    C.norm = take(NRM_SLOTS);
+    C.scal = take(4);
+    C.sp = (StepDev*)take(sizeof(StepDev) / 8 + 1);
+    BCHK(hipHostMalloc((void**)&C.h_norm, 8 * sizeof(double), hipHostMallocDefault));
+    for (auto& e : C.ev) BCHK(hipEventCreate(&e));
+    return CS_OK;
+}
+
+void launch_factor(BigCircuit& C, hipStream_t s) {
+    const BigDev* P = C.d_dev;
+    for (auto& Lv : C.levels) {
+        if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
+        if (Lv.orig_e > Lv.orig_b)
+            hipLaunchKernelGGL(k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval,
+                               C.pool, Lv.orig_b, Lv.orig_e);
+        if (Lv.small_n)
+            hipLaunchKernelGGL(k_mf_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
+                               C.d_fronts + Lv.small_b);
+        for (auto& pn : Lv.panels) {
+            hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
+            if (pn.trsm_n)
+                hipLaunchKernelGGL(k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_trsm + pn.trsm_b);
+            if (pn.gemm_n)
+                hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+        }
+    }
+}
+
+void launch_solve(BigCircuit& C, hipStream_t s) {
+    const BigDev* P = C.d_dev;
+    for (auto& Lv : C.levels)
+        hipLaunchKernelGGL(k_mf_fwd, dim3(Lv.fr_n), dim3(kBT), Lv.solve_lds, s, P, C.pool, C.x, C.uvec,
+                           C.d_fronts + Lv.fr_b);
+    for (auto it = C.levels.rbegin(); it != C.levels.rend(); ++it)
+        hipLaunchKernelGGL(k_mf_bwd, dim3(it->fr_n), dim3(kBT), it->solve_lds, s, P, C.pool, C.dinv, C.x,
+                           C.d_fronts + it->fr_b);
+}
+
+int ensure_graph(BigCircuit& C, hipStream_t s, std::string& err) {
+    if (C.exec) return CS_OK;
+    const BigDev* P = C.d_dev;
+    const int gr = grid_for(C.B.n);
+    const int ge = grid_for(C.B.base.n_mos + C.B.base.n_src);
+    BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_eval, dim3(ge), dim3(kBT), 0, s, P, C.p, C.u, C.sp, C.st);
+    hipLaunchKernelGGL(k_assemble, dim3(gr), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, C.qx, C.st, C.u0,
+                       C.Aval, C.x, C.part);
+    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(kBT), 0, s, C.part, gr, C.norm);
+    launch_factor(C, s);
+    launch_solve(C, s);
+    hipLaunchKernelGGL(k_update, dim3(gr), dim3(kBT), 0, s, P, C.sp, C.x, C.u, C.norm, C.part);
+    hipLaunchKernelGGL(k_upd_reduce, dim3(1), dim3(kBT), 0, s, P, C.sp, C.part, gr, C.norm);
+    (void)hipMemcpyAsync(C.h_norm, C.norm, NRM_SLOTS * sizeof(double), hipMemcpyDeviceToHost, s);
+    BCHK(hipStreamEndCapture(s, &C.graph));
+    BCHK(hipGraphInstantiate(&C.exec, C.graph, nullptr, nullptr, 0));
+    return CS_OK;
+}
+
+int set_params(BigCircuit& C, hipStream_t s, const double* p, std::string& err) {
+    const Plan& P = C.B.base;
+    BCHK(hipMemcpyAsync(C.p, p, sizeof(double) * P.lay.n_params, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_lin_a, dim3(grid_for(P.n_lin)), dim3(kBT), 0, s, C.d_dev, C.p, C.linbuf);
+    hipLaunchKernelGGL(k_lin_b, dim3(grid_for(P.nnz_nat)), dim3(kBT), 0, s, C.d_dev, C.linbuf, C.Glin, C.Clin);
+    BCHK(hipGetLastError());
+    return CS_OK;
+}
+
+int run_iter(BigCircuit& C, hipStream_t s, std::string& err) {
+    BCHK(hipGraphLaunch(C.exec, s));
+    BCHK(hipStreamSynchronize(s));
+    return CS_OK;
+}
+
+int set_step(BigCircuit& C, hipStream_t s, const StepDev& sp, std::string& err) {
+    BCHK(hipMemcpyAsync(C.sp, &sp, sizeof(StepDev), hipMemcpyHostToDevice, s));
+    return CS_OK;
+}
+
+int init_point(BigCircuit& C, hipStream_t s, const double* u0, const cs_tran_opts& o, bool uic,
+               cs_tran_stats* st, std::string& err) {
+    const int32_t n = C.B.n;
+    BCHK(hipMemsetAsync(C.u, 0, sizeof(double) * n, s));
+    if (uic) {
+        BCHK(hipMemcpyAsync(C.u, u0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        BCHK(hipMemcpyAsync(C.u0, u0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    }
+    StepDev sp{};
+    sp.a0 = 0.0; sp.t = o.t_start; sp.rtol = o.reltol; sp.atol = o.abstol; sp.tol = o.dc_abstol;
+    sp.mode = uic ? CS_MODE_TRAN : CS_MODE_DCOP;
+    sp.kind = uic ? STEP_UIC : STEP_DC;
+    int rc = set_step(C, s, sp, err);
+    if (rc) return rc;
+    const int maxit = uic ? 50 : 100;
+    for (int it = 0; it < maxit; it++) {
+        if ((rc = run_iter(C, s, err))) return rc;
+        if (C.h_norm[NRM_DONE] == 1.0) return CS_OK;
+        const double r = C.h_norm[NRM_RES];
+        if (!(r == r) || std::isinf(r)) break;
+        if (st) st->newton_iters++;
+    }
+    return CS_E_NOCONV;
+}
+
+}  // namespace
+
+int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::string& err) {
+    (void)s;
+    auto* C = new BigCircuit();
+    int rc = build_bigplan(nl, C->B, err);
+    if (rc == CS_OK) rc = upload(*C, err);
+    if (rc != CS_OK) {
+        big_destroy(C);
+        return rc;
+    }
+    *out = C;
+    return CS_OK;
+}
+
+void big_destroy(BigCircuit* C) {
+    if (!C) return;
+    if (C->exec) (void)hipGraphExecDestroy(C->exec);
+    if (C->graph) (void)hipGraphDestroy(C->graph);
+    for (auto& e : C->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (C->h_norm) (void)hipHostFree(C->h_norm);
+    if (C->wbuf) (void)hipFree(C->wbuf);
+    if (C->tbuf) (void)hipFree(C->tbuf);
+    if (C->dbuf) (void)hipFree(C->dbuf);
+    if (C->d_dev) (void)hipFree(C->d_dev);
+    delete C;
+}
+
+const BigPlan& big_plan(const BigCircuit* C) { return C->B; }
+
+void big_info(const BigPlan& B, cs_big_info* info) {
+    info->n = B.n;
+    info->n_super = B.ns;
+    info->n_levels = (int32_t)B.lev_ptr.size() - 1;
+    info->max_front = B.max_m;
+    info->max_pivots = B.max_k;
+    info->n_hub = B.n_hub;
+    info->pool = B.pool;
+    info->nnz_factor = B.nnz_factor;
+    info->flops = B.flops;
+}
+
+int big_set_probes(BigCircuit* C, const std::vector<int32_t>& idx, std::string& err) {
+    set_probes(C->B.base, idx);
+    if (!idx.empty())
+        BCHK(hipMemcpy(C->d_probe, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+    return CS_OK;
+}
+
+void big_last_times(const BigCircuit* C, double* f, double* s) {
+    if (f) *f = C->factor_ms;
+    if (s) *s = C->solve_ms;
+}
+
+int big_dcop(BigCircuit* C, hipStream_t s, const double* p, const cs_tran_opts* o, double* u_out,
+             cs_tran_stats* st, std::string& err) {
+    int rc;
+    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_graph(*C, s, err))) return rc;
+    cs_tran_stats stt{};
+    rc = init_point(*C, s, nullptr, *o, false, &stt, err);
+    if (u_out) BCHK(hipMemcpy(u_out, C->u, sizeof(double) * C->B.n, hipMemcpyDeviceToHost));
+    stt.retcode = rc;
+    stt.t_end = o->t_start;
+    if (st) *st = stt;
+    if (rc == CS_E_NOCONV) err = "dc operating point did not converge";
+    return rc;
+}
+
+// Transient with the step rules of rules.h; the history lives on the device
+// and the host reads two scalars per Newton iteration and one per step.
+int big_tran(BigCircuit* C, hipStream_t s, const double* p, const double* u0, const cs_tran_opts* o,
+             double* wave, double* u_final, cs_tran_stats* st_out, std::string& err) {
+    const BigPlan& B = C->B;
+    const Plan& P = B.base;
+    const int32_t n = B.n;
+    const bool uic = o->init == CS_INIT_UIC;
+    if (uic && !u0) { err = "UIC initialisation needs u0"; return CS_E_INVALID; }
+    if (uic && !P.uic_ok) { err = "UIC: a charge-carrying row could not be kept on its own pivot"; return CS_E_INVALID; }
+    if (o->n_out > 0 && !(o->dt_out > 0)) { err = "dt_out must be > 0"; return CS_E_INVALID; }
+    int rc;
+    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_graph(*C, s, err))) return rc;
+    cs_tran_stats st{};
+    rc = init_point(*C, s, u0, *o, uic, &st, err);
+    const int gr = grid_for(n);
+    const int n_probe = (int)P.probe.size();
+    double* d_wave = nullptr;
+    if (wave && o->n_out > 0) BCHK(hipMalloc(&d_wave, sizeof(double) * (size_t)o->n_out * n_probe));
+    const StepDefaults sd = step_defaults(*o);
+    double t = o->t_start;
+    int kout = 0;
+    if (rc == CS_OK) {
+        BCHK(hipMemcpyAsync(C->H[0], C->u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_charges, dim3(gr), dim3(kBT), 0, s, C->d_dev, C->Clin, C->u, C->Q[0]);
+        double T0 = o->t_start, T1 = 0.0, T2 = 0.0;
+        int npts = 1;
+        double h = sd.dt_init;
+        if (d_wave)
+            while (kout < o->n_out && o->t_start + kout * o->dt_out <= o->t_start + 1e-9 * o->dt_out) {
+                hipLaunchKernelGGL(k_wave, dim3(grid_for(n_probe)), dim3(kBT), 0, s, n_probe, C->d_probe, C->H[0],
+                                   C->H[0], C->H[0], 1.0, 0.0, 0.0, d_wave + (size_t)kout * n_probe);
+                kout++;
+            }
+        std::vector<double> ph(p, p + P.lay.n_params);
+        StepDev sp{};
+        sp.rtol = o->reltol; sp.atol = o->abstol; sp.tol = o->dc_abstol;
+        sp.mode = CS_MODE_TRAN; sp.kind = STEP_TRAN;
+        while (t < o->t_stop) {
+            if (st.steps + st.rejected >= o->max_steps) { rc = CS_E_MAXSTEPS; break; }
+            double tb = INFINITY;
+            for (auto& sdsc : P.src) {
+                const double bp = wave_next_bp(sdsc.kind, ph.data() + sdsc.off, sdsc.len, t);
+                if (bp < tb) tb = bp;
+            }
+            const bool srcbp = tb < o->t_stop;
+            if (!srcbp) tb = o->t_stop;
+            double hh = h < sd.dt_max ? h : sd.dt_max;
+            int hit;
+            const double tn = clip_step(t, hh, tb, hit);
+            const int k = bdf_order(npts, o->max_order);
+            double a0, a1, a2, w0, w1, w2;
+            bdf_coeffs(k, tn, T0, T1, a0, a1, a2);
+            const int np = npts < k + 1 ? npts : k + 1;
+            pred_weights(np, tn, T0, T1, T2, w0, w1, w2);
+            hipLaunchKernelGGL(k_predict, dim3(gr), dim3(kBT), 0, s, n, np, k, w0, w1, w2, a1, a2, C->H[0],
+                               C->H[1], C->H[2], C->Q[0], C->Q[1], C->up, C->u, C->qx);
+            sp.a0 = a0;
+            sp.t = tn;
+            if ((rc = set_step(*C, s, sp, err))) break;
+            bool conv = false;
+            for (int it = 0; it < o->newton_maxit; it++) {
+                if ((rc = run_iter(*C, s, err))) break;
+                st.newton_iters++;
+                const double nr = C->h_norm[NRM_WRMS];
+                if (!(nr == nr)) break;
+                if (nr <= o->newton_tol) { conv = true; break; }
+            }
+            if (rc) break;
+            if (!conv) {
+                st.rejected++;
+                h = 0.25 * hh;
+                if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
+                continue;
+            }
+            hipLaunchKernelGGL(k_charges, dim3(gr), dim3(kBT), 0, s, C->d_dev, C->Clin, C->u, C->qn);
+            double fac = 2.0;
+            if (npts >= 2) {
+                const double c = lte_coeff(k, tn, T0, T1, T2, a0, a1, a2);
+                hipLaunchKernelGGL(k_lte, dim3(gr), dim3(kBT), 0, s, n, c, o->reltol, o->abstol, C->u, C->up,
+                                   C->H[0], C->part);
+                hipLaunchKernelGGL(k_sum_reduce, dim3(1), dim3(kBT), 0, s, C->part, gr, C->scal);
+                BCHK(hipMemcpyAsync(C->h_norm + 4, C->scal, sizeof(double), hipMemcpyDeviceToHost, s));
+                BCHK(hipStreamSynchronize(s));
+                const double err_ = std::sqrt(C->h_norm[4] / n);
+                if (!(err_ <= 1.0)) {
+                    st.rejected++;
+                    h = hh * reject_factor(err_, k);
+                    if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
+                    continue;
+                }
+                fac = accept_factor(err_, k);
+            }
+            // accept: rotate the history buffers, u becomes the newest point
+            std::swap(C->H[2], C->H[1]);
+            std::swap(C->H[1], C->H[0]);
+            BCHK(hipMemcpyAsync(C->H[0], C->u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            std::swap(C->Q[1], C->Q[0]);
+            std::swap(C->Q[0], C->qn);
+            T2 = T1; T1 = T0; T0 = tn;
+            if (npts < 3) npts++;
+            st.steps++;
+            if (d_wave) {
+                while (kout < o->n_out) {
+                    const double tq = o->t_start + kout * o->dt_out;
+                    if (tq > tn + 1e-9 * o->dt_out) break;
+                    double l0, l1, l2 = 0.0;
+                    if (npts >= 3) {
+                        l0 = (tq - T1) * (tq - T2) / ((T0 - T1) * (T0 - T2));
+                        l1 = (tq - T0) * (tq - T2) / ((T1 - T0) * (T1 - T2));
+                        l2 = (tq - T0) * (tq - T1) / ((T2 - T0) * (T2 - T1));
+                    } else {
+                        const double sl = (tq - T1) / (T0 - T1);
+                        l0 = sl;
+                        l1 = 1.0 - sl;
+                    }
+                    hipLaunchKernelGGL(k_wave, dim3(grid_for(n_probe)), dim3(kBT), 0, s, n_probe, C->d_probe,
+                                       C->H[0], C->H[1], C->H[2], l0, l1, l2, d_wave + (size_t)kout * n_probe);
+                    kout++;
+                }
+            }
+            t = tn;
+            h = hh * fac;
+            if (hit && srcbp) {
+                npts = 1;
+                h = 0.1 * h;
+                if (h < sd.dt_min) h = sd.dt_min;
+            }
+        }
+    }
+    st.t_end = t;
+    st.retcode = rc;
+    if (d_wave) {
+        (void)hipStreamSynchronize(s);
+        (void)hipMemcpy(wave, d_wave, sizeof(double) * (size_t)kout * n_probe, hipMemcpyDeviceToHost);
+        (void)hipFree(d_wave);
+    }
+    if (u_final) {
+        (void)hipStreamSynchronize(s);
+        (void)hipMemcpy(u_final, st.steps > 0 ? C->H[0] : C->u, sizeof(double) * n, hipMemcpyDeviceToHost);
+    }
+    if (st_out) *st_out = st;
+    if (rc == CS_E_DTMIN) err = "transient: step size fell below dt_min";
+    else if (rc == CS_E_NOCONV) err = "transient: initial point did not converge";
+    else if (rc == CS_E_MAXSTEPS) err = "transient: max_steps reached";
+    return rc;
+}
+
+// ---- DAE interface ------------------------------------------------------------------------------
+int big_residual(BigCircuit* C, hipStream_t s, double* out, const double* du, const double* u,
+                 const double* p, double t, int32_t mode, std::string& err) {
+    const int32_t n = C->B.n;
+    int rc;
+    if ((rc = set_params(*C, s, p, err))) return rc;
+    BCHK(hipMemcpyAsync(C->u, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    StepDev sp{};
+    sp.a0 = 0.0; sp.t = t; sp.mode = mode;
+    sp.kind = mode == CS_MODE_DCOP ? STEP_DC : STEP_TRAN;
+    if (mode != CS_MODE_DCOP) {
+        // the charge term d/dt q(u) = C du enters as qx with a0 = 0
+        if (du) BCHK(hipMemcpyAsync(C->up, du, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        else BCHK(hipMemsetAsync(C->up, 0, sizeof(double) * n, s));
+        hipLaunchKernelGGL(k_charges, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Clin, C->up, C->qx);
+    }
+    if ((rc = set_step(*C, s, sp, err))) return rc;
+    hipLaunchKernelGGL(k_eval, dim3(grid_for(C->B.base.n_mos + C->B.base.n_src)), dim3(kBT), 0, s, C->d_dev, C->p,
+                       C->u, C->sp, C->st);
+    hipLaunchKernelGGL(k_assemble, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Glin, C->Clin, C->sp, C->u,
+                       C->qx, C->st, nullptr, C->Aval, C->x, C->part);
+    // x[prow[r]] = -F(r)
+    hipLaunchKernelGGL(k_perm_out, dim3(grid_for(n)), dim3(kBT), 0, s, n, C->hd.prow, C->x, C->up);
+    std::vector<double> h(n);
+    BCHK(hipMemcpyAsync(h.data(), C->up, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    BCHK(hipStreamSynchronize(s));
+    for (int32_t i = 0; i < n; i++) out[i] = -h[i];
+    return CS_OK;
+}
+
+int big_jacobian(BigCircuit* C, hipStream_t s, double* vals, const double* du, const double* u,
+                 const double* p, double gamma, double t, int32_t mode, std::string& err) {
+    (void)du;
+    const int32_t n = C->B.n;
+    int rc;
+    if ((rc = set_params(*C, s, p, err))) return rc;
+    BCHK(hipMemcpyAsync(C->u, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    StepDev sp{};
+    sp.a0 = gamma; sp.t = t; sp.mode = mode;
+    sp.kind = mode == CS_MODE_DCOP ? STEP_DC : STEP_TRAN;
+    if ((rc = set_step(*C, s, sp, err))) return rc;
+    hipLaunchKernelGGL(k_eval, dim3(grid_for(C->B.base.n_mos + C->B.base.n_src)), dim3(kBT), 0, s, C->d_dev, C->p,
+                       C->u, C->sp, C->st);
+    hipLaunchKernelGGL(k_assemble, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Glin, C->Clin, C->sp, C->u,
+                       nullptr, C->st, nullptr, C->Aval, nullptr, nullptr);
+    BCHK(hipMemcpyAsync(vals, C->Aval, sizeof(double) * C->B.base.nnz_nat, hipMemcpyDeviceToHost, s));
+    BCHK(hipStreamSynchronize(s));
+    return CS_OK;
+}
+
+int big_lu_refactor(BigCircuit* C, hipStream_t s, const double* vals, std::string& err) {
+    BCHK(hipMemcpyAsync(C->Aval, vals, sizeof(double) * C->B.base.nnz_nat, hipMemcpyHostToDevice, s));
+    BCHK(hipEventRecord(C->ev[0], s));
+    launch_factor(*C, s);
+    BCHK(hipEventRecord(C->ev[1], s));
+    BCHK(hipGetLastError());
+    BCHK(hipEventSynchronize(C->ev[1]));
+    float ms = 0.f;
+    BCHK(hipEventElapsedTime(&ms, C->ev[0], C->ev[1]));
+    C->factor_ms = ms;
+    std::vector<double> d(C->B.n);
+    BCHK(hipMemcpy(d.data(), C->dinv, sizeof(double) * C->B.n, hipMemcpyDeviceToHost));
+    for (int32_t k = 0; k < C->B.n; k++)
+        if (!std::isfinite(d[k])) {
+            err = "zero pivot at step " + std::to_string(k);
+            return CS_E_SINGULAR;
+        }
+    return CS_OK;
+}
+
+int big_lu_solve(BigCircuit* C, hipStream_t s, double* x, std::string& err) {
+    const int32_t n = C->B.n;
+    BCHK(hipMemcpyAsync(C->up, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n)), dim3(kBT), 0, s, n, C->hd.prow, C->up, C->x);
+    BCHK(hipEventRecord(C->ev[2], s));
+    launch_solve(*C, s);
+    BCHK(hipEventRecord(C->ev[3], s));
+    hipLaunchKernelGGL(k_perm_out, dim3(grid_for(n)), dim3(kBT), 0, s, n, C->hd.pcol, C->x, C->up);
+    BCHK(hipMemcpyAsync(x, C->up, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    BCHK(hipStreamSynchronize(s));
+    BCHK(hipGetLastError());
+    float ms = 0.f;
+    BCHK(hipEventElapsedTime(&ms, C->ev[2], C->ev[3]));
+    C->solve_ms = ms;
+    return CS_OK;
+}
+
+}  // namespace cedar

@@ -1,0 +1,470 @@
+// bigplan.cpp — host symbolic analysis of the supernodal multifrontal path.
+//
+// The reference has no counterpart: it factors the DAE Jacobian densely in
+// IDA (src/sweeps.jl:456), which is out of reach at 10^5 unknowns; this is
+// the sparse direct solver that makes such circuits run on the device.
+#include <algorithm>
+#include <numeric>
+
+#include "bigplan.h"
+
+namespace cedar {
+
+namespace {
+
+struct Graph {
+    std::vector<int32_t> xadj, adj;
+    int32_t n() const { return (int32_t)xadj.size() - 1; }
+};
+
+// Nested dissection by recursive breadth-first bisection.  Each subgraph is
+// split by the middle level of a breadth-first level structure rooted at a
+// pseudo-peripheral vertex (on a mesh: a straight-ish cut of about the side
+// length); separator vertices with no neighbour on the far side are moved
+// back.  Elimination order: part A, part B, separator.
+class Dissector {
+public:
+    Dissector(const Graph& g, int32_t leaf) : g_(g), leaf_(leaf), mark_(g.n(), -1), lvl_(g.n(), -1) {}
+
+    void run(std::vector<int32_t> verts, std::vector<int32_t>& order) {
+        struct Item {
+            std::vector<int32_t> v;
+            bool append;
+        };
+        std::vector<Item> stack;
+        stack.push_back({std::move(verts), false});
+        while (!stack.empty()) {
+            Item it = std::move(stack.back());
+            stack.pop_back();
+            if (it.append || (int32_t)it.v.size() <= leaf_) {
+                order.insert(order.end(), it.v.begin(), it.v.end());
+                continue;
+            }
+            std::vector<int32_t> A, B, S;
+            std::vector<std::vector<int32_t>> comps;
+            split(it.v, A, B, S, comps);
+            if (!comps.empty()) {
+                for (auto c = comps.rbegin(); c != comps.rend(); ++c) stack.push_back({std::move(*c), false});
+                continue;
+            }
+            if (A.empty() || B.empty()) {   // no useful cut: keep as one block
+                order.insert(order.end(), it.v.begin(), it.v.end());
+                continue;
+            }
+            stack.push_back({std::move(S), true});
+            stack.push_back({std::move(B), false});
+            stack.push_back({std::move(A), false});
+        }
+    }
+
+private:
+    const Graph& g_;
+    int32_t leaf_;
+    std::vector<int32_t> mark_, lvl_;
+    int32_t epoch_ = 0;
+
+    // BFS over the vertices marked `id` from root; returns the visit order
+    // and leaves lvl_ set for the visited vertices.
+    void bfs(int32_t root, int32_t id, std::vector<int32_t>& seen) {
+        seen.clear();
+        seen.push_back(root);
+        lvl_[root] = 0;
+        for (size_t h = 0; h < seen.size(); h++) {
+            const int32_t v = seen[h];
+            for (int32_t e = g_.xadj[v]; e < g_.xadj[v + 1]; e++) {
+                const int32_t w = g_.adj[e];
+                if (mark_[w] != id || lvl_[w] >= 0) continue;
+                lvl_[w] = lvl_[v] + 1;
+                seen.push_back(w);
+            }
+        }
+    }
+    void clear_lvl(const std::vector<int32_t>& seen) {
+        for (int32_t v : seen) lvl_[v] = -1;
+    }
+
+    void split(const std::vector<int32_t>& V, std::vector<int32_t>& A, std::vector<int32_t>& B,
+               std::vector<int32_t>& S, std::vector<std::vector<int32_t>>& comps) {
+        const int32_t id = epoch_++;
+        for (int32_t v : V) mark_[v] = id;
+        std::vector<int32_t> seen;
+        bfs(V[0], id, seen);
+        if (seen.size() < V.size()) {   // disconnected: dissect the components separately
+            std::vector<int32_t> comp_of_seen = seen;
+            clear_lvl(seen);
+            comps.push_back(std::move(comp_of_seen));
+            for (int32_t v : comps.back()) mark_[v] = -2 - id;   // taken
+            for (int32_t v : V) {
+                if (mark_[v] != id) continue;
+                bfs(v, id, seen);
+                clear_lvl(seen);
+                for (int32_t w : seen) mark_[w] = -2 - id;
+                comps.push_back(seen);
+            }
+            return;
+        }
+        // pseudo-peripheral root
+        int32_t root = V[0], ecc = lvl_[seen.back()];
+        for (int rep = 0; rep < 4; rep++) {
+            const int32_t last = lvl_[seen.back()];
+            int32_t cand = seen.back(), cdeg = INT32_MAX;
+            for (auto it = seen.rbegin(); it != seen.rend() && lvl_[*it] == last; ++it) {
+                const int32_t d = g_.xadj[*it + 1] - g_.xadj[*it];
+                if (d < cdeg) { cdeg = d; cand = *it; }
+            }
+            clear_lvl(seen);
+            bfs(cand, id, seen);
+            const int32_t e2 = lvl_[seen.back()];
+            if (e2 <= ecc) {
+                clear_lvl(seen);
+                bfs(root, id, seen);
+                break;
+            }
+            root = cand;
+            ecc = e2;
+        }
+        const int32_t depth = lvl_[seen.back()];
+        if (depth < 2) {
+            clear_lvl(seen);
+            A.clear();
+            B.clear();
+            return;
+        }
+        // middle level by count
+        std::vector<int32_t> cnt(depth + 1, 0);
+        for (int32_t v : seen) cnt[lvl_[v]]++;
+        int32_t sep = 0, acc = 0;
+        const int32_t half = (int32_t)V.size() / 2;
+        for (int32_t l = 0; l <= depth; l++) {
+            if (acc + cnt[l] >= half) { sep = l; break; }
+            acc += cnt[l];
+        }
+        if (sep == 0) sep = 1;
+        if (sep == depth) sep = depth - 1;
+        for (int32_t v : seen) {
+            const int32_t l = lvl_[v];
+            if (l < sep) A.push_back(v);
+            else if (l > sep) B.push_back(v);
+            else {
+                bool far = false;
+                for (int32_t e = g_.xadj[v]; e < g_.xadj[v + 1] && !far; e++) {
+                    const int32_t w = g_.adj[e];
+                    far = mark_[w] == id && lvl_[w] == sep + 1;
+                }
+                if (far) S.push_back(v);
+                else A.push_back(v);
+            }
+        }
+        clear_lvl(seen);
+    }
+};
+
+int32_t lower_pos(const int32_t* b, const int32_t* e, int32_t x) {
+    const int32_t* it = std::lower_bound(b, e, x);
+    return (it != e && *it == x) ? (int32_t)(it - b) : -1;
+}
+
+}  // namespace
+
+int build_bigplan(const cs_netlist* nl, BigPlan& B, std::string& err) {
+    B = BigPlan();
+    Plan& P = B.base;
+    Stamps S;
+    int rc = stamp_netlist(nl, P, S, err);
+    if (rc) return rc;
+    PivotMatch M;
+    if ((rc = match_pivots(P, S, M, err))) return rc;
+    P.n_singletons = M.n_singletons;
+    const int32_t n = P.n;
+    B.n = n;
+
+    // matched row of every column
+    std::vector<int32_t> mrow(n, -1);
+    for (size_t q = 0; q < M.pcol_of_k.size(); q++) mrow[M.pcol_of_k[q]] = M.prow_of_k[q];
+    for (int32_t c : M.rem_cols) mrow[c] = M.match_col[c];
+    for (int32_t c = 0; c < n; c++)
+        if (mrow[c] < 0) { err = "internal: unmatched column"; return CS_E_SINGULAR; }
+
+    // graph of M + M^T over columns (M row c = natural row mrow[c])
+    Graph g;
+    {
+        std::vector<std::vector<int32_t>> nb(n);
+        for (int32_t c = 0; c < n; c++)
+            for (int32_t c2 : S.rowc[mrow[c]])
+                if (c2 != c) {
+                    nb[c].push_back(c2);
+                    nb[c2].push_back(c);
+                }
+        g.xadj.assign(n + 1, 0);
+        for (int32_t c = 0; c < n; c++) {
+            auto& v = nb[c];
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            g.xadj[c + 1] = g.xadj[c] + (int32_t)v.size();
+        }
+        g.adj.reserve(g.xadj[n]);
+        for (auto& v : nb) g.adj.insert(g.adj.end(), v.begin(), v.end());
+    }
+
+    // hubs last, nested dissection of the rest
+    std::vector<int32_t> order;
+    order.reserve(n);
+    {
+        const double avg = n > 0 ? (double)g.xadj[n] / n : 0.0;
+        const int32_t hub_deg = std::max<int32_t>(64, (int32_t)(16 * avg));
+        std::vector<char> hub(n, 0);
+        std::vector<int32_t> rest, hubs;
+        for (int32_t c = 0; c < n; c++) {
+            if (g.xadj[c + 1] - g.xadj[c] > hub_deg) { hub[c] = 1; hubs.push_back(c); }
+            else rest.push_back(c);
+        }
+        // drop hub edges from the dissected graph
+        Graph h;
+        h.xadj.assign(n + 1, 0);
+        for (int32_t c = 0; c < n; c++) {
+            int32_t d = 0;
+            if (!hub[c])
+                for (int32_t e = g.xadj[c]; e < g.xadj[c + 1]; e++) d += !hub[g.adj[e]];
+            h.xadj[c + 1] = h.xadj[c] + d;
+        }
+        h.adj.reserve(h.xadj[n]);
+        for (int32_t c = 0; c < n; c++)
+            if (!hub[c])
+                for (int32_t e = g.xadj[c]; e < g.xadj[c + 1]; e++)
+                    if (!hub[g.adj[e]]) h.adj.push_back(g.adj[e]);
+        if (!rest.empty()) {
+            Dissector nd(h, 32);
+            nd.run(rest, order);
+        }
+        order.insert(order.end(), hubs.begin(), hubs.end());
+        B.n_hub = (int32_t)hubs.size();
+    }
+    if ((int32_t)order.size() != n) { err = "internal: ordering lost vertices"; return CS_E_INVALID; }
+    B.colnat = order;
+    B.rownat.resize(n);
+    B.pcol.assign(n, -1);
+    B.prow.assign(n, -1);
+    for (int32_t k = 0; k < n; k++) {
+        B.rownat[k] = mrow[order[k]];
+        B.pcol[order[k]] = k;
+        B.prow[B.rownat[k]] = k;
+    }
+    P.rownat = B.rownat;
+    P.prow = B.prow;
+    P.pcol = B.pcol;
+    P.uic_ok = true;
+    for (int32_t r = 0; r < n; r++)
+        if (P.is_diff[r] && B.pcol[r] != B.prow[r]) P.uic_ok = false;
+
+    // elimination tree (Liu, with path compression)
+    std::vector<int32_t> parent(n, -1), anc(n, -1);
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t v = order[k];
+        for (int32_t e = g.xadj[v]; e < g.xadj[v + 1]; e++) {
+            int32_t j = B.pcol[g.adj[e]];
+            if (j >= k) continue;
+            while (anc[j] != -1 && anc[j] != k) {
+                const int32_t nx = anc[j];
+                anc[j] = k;
+                j = nx;
+            }
+            if (anc[j] == -1) {
+                anc[j] = k;
+                parent[j] = k;
+            }
+        }
+    }
+    // column structures of L (rows > k), children merged into parents
+    std::vector<std::vector<int32_t>> cs(n);
+    std::vector<int32_t> nchild(n, 0);
+    {
+        std::vector<std::vector<int32_t>> kids(n);
+        for (int32_t k = 0; k < n; k++)
+            if (parent[k] >= 0) { kids[parent[k]].push_back(k); nchild[parent[k]]++; }
+        std::vector<int32_t> tmp;
+        for (int32_t k = 0; k < n; k++) {
+            std::vector<int32_t> s;
+            const int32_t v = order[k];
+            for (int32_t e = g.xadj[v]; e < g.xadj[v + 1]; e++) {
+                const int32_t j = B.pcol[g.adj[e]];
+                if (j > k) s.push_back(j);
+            }
+            std::sort(s.begin(), s.end());
+            for (int32_t c : kids[k]) {
+                tmp.clear();
+                const auto& sc = cs[c];
+                auto b = std::upper_bound(sc.begin(), sc.end(), k);
+                std::set_union(s.begin(), s.end(), b, sc.end(), std::back_inserter(tmp));
+                s.swap(tmp);
+            }
+            cs[k].swap(s);
+        }
+    }
+    // fundamental supernodes, then small chains merged into their parent
+    std::vector<int32_t> first;
+    for (int32_t k = 0; k < n; k++) {
+        const bool cont = k > 0 && parent[k - 1] == k && nchild[k] == 1 &&
+                          cs[k - 1].size() == cs[k].size() + 1;
+        if (!cont) first.push_back(k);
+    }
+    first.push_back(n);
+    {
+        const int32_t kRelax = 16;
+        std::vector<int32_t> merged;
+        merged.push_back(0);
+        for (size_t s = 0; s + 1 < first.size(); s++) {
+            const int32_t f = merged.back(), e = first[s + 1];
+            // merge this supernode [f, e) with the next one when the next one
+            // is its parent and starts at e
+            const int32_t last = e - 1;
+            const bool parent_next = e < n && !cs[last].empty() && cs[last][0] == e;
+            const int32_t next_end = s + 2 < first.size() ? first[s + 2] : n;
+            // no zeros: this block's update rows are exactly the next block's
+            // columns and update rows (a dense separator split by a second child)
+            const bool exact = parent_next && cs[last].size() == (size_t)(next_end - e) + cs[next_end - 1].size();
+            if (parent_next && (exact || (next_end - f) <= kRelax)) continue;   // extend into the next
+            merged.push_back(e);
+        }
+        first.swap(merged);
+    }
+    const int32_t ns = (int32_t)first.size() - 1;
+    B.ns = ns;
+    B.sn_first = first;
+    std::vector<int32_t> sn_of(n);
+    for (int32_t s = 0; s < ns; s++)
+        for (int32_t k = first[s]; k < first[s + 1]; k++) sn_of[k] = s;
+    B.sn_m.resize(ns);
+    B.sn_parent.assign(ns, -1);
+    B.sn_rptr.assign(ns + 1, 0);
+    B.sn_off.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; s++) {
+        const auto& u = cs[first[s + 1] - 1];
+        const int32_t k = first[s + 1] - first[s];
+        B.sn_m[s] = k + (int32_t)u.size();
+        B.sn_rptr[s + 1] = B.sn_rptr[s] + (int32_t)u.size();
+        B.sn_rows.insert(B.sn_rows.end(), u.begin(), u.end());
+        if (!u.empty()) B.sn_parent[s] = sn_of[u[0]];
+        B.sn_off[s + 1] = B.sn_off[s] + (int64_t)B.sn_m[s] * B.sn_m[s];
+        B.max_m = std::max(B.max_m, B.sn_m[s]);
+        B.max_k = std::max(B.max_k, k);
+        const double m = B.sn_m[s];
+        for (int32_t p = 0; p < k; p++) {
+            const double t = m - p - 1;
+            B.flops += t + 2.0 * t * t;
+        }
+        B.nnz_factor += (int64_t)B.sn_m[s] * B.sn_m[s] - (int64_t)u.size() * u.size();
+    }
+    B.pool = B.sn_off[ns];
+    // every row of a front must lie in the front of the parent chain
+    auto local = [&](int32_t s, int32_t k) -> int32_t {
+        if (k >= first[s] && k < first[s + 1]) return k - first[s];
+        const int32_t* b = B.sn_rows.data() + B.sn_rptr[s];
+        const int32_t* e = B.sn_rows.data() + B.sn_rptr[s + 1];
+        const int32_t q = lower_pos(b, e, k);
+        return q < 0 ? -1 : (first[s + 1] - first[s]) + q;
+    };
+    // children, levels
+    B.ch_ptr.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; s++)
+        if (B.sn_parent[s] >= 0) B.ch_ptr[B.sn_parent[s] + 1]++;
+    for (int32_t s = 0; s < ns; s++) B.ch_ptr[s + 1] += B.ch_ptr[s];
+    B.ch_sn.resize(B.ch_ptr[ns]);
+    {
+        std::vector<int32_t> fill(B.ch_ptr.begin(), B.ch_ptr.end() - 1);
+        for (int32_t s = 0; s < ns; s++)
+            if (B.sn_parent[s] >= 0) B.ch_sn[fill[B.sn_parent[s]]++] = s;
+    }
+    B.sn_level.assign(ns, 0);
+    int32_t nlev = 0;
+    for (int32_t s = 0; s < ns; s++) {
+        int32_t l = 0;
+        for (int32_t q = B.ch_ptr[s]; q < B.ch_ptr[s + 1]; q++) l = std::max(l, B.sn_level[B.ch_sn[q]] + 1);
+        B.sn_level[s] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    B.lev_ptr.assign(nlev + 1, 0);
+    for (int32_t s = 0; s < ns; s++) B.lev_ptr[B.sn_level[s] + 1]++;
+    for (int32_t l = 0; l < nlev; l++) B.lev_ptr[l + 1] += B.lev_ptr[l];
+    B.lev_sn.resize(ns);
+    {
+        std::vector<int32_t> fill(B.lev_ptr.begin(), B.lev_ptr.end() - 1);
+        for (int32_t s = 0; s < ns; s++) B.lev_sn[fill[B.sn_level[s]]++] = s;
+    }
+    // relative maps of each child's update rows in its parent's front
+    B.rel_ptr.assign(ns + 1, 0);
+    B.inv_ptr.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; s++) {
+        const int32_t r = B.sn_rptr[s + 1] - B.sn_rptr[s];
+        B.rel_ptr[s + 1] = B.rel_ptr[s] + r;
+        B.inv_ptr[s + 1] = B.inv_ptr[s] + (B.sn_parent[s] >= 0 ? B.sn_m[B.sn_parent[s]] : 0);
+    }
+    B.rel.resize(B.rel_ptr[ns]);
+    B.inv_total = B.inv_ptr[ns];
+    B.inv.assign(B.inv_total, -1);
+    for (int32_t s = 0; s < ns; s++) {
+        const int32_t p = B.sn_parent[s];
+        if (p < 0) continue;
+        for (int32_t a = 0; a < B.sn_rptr[s + 1] - B.sn_rptr[s]; a++) {
+            const int32_t loc = local(p, B.sn_rows[B.sn_rptr[s] + a]);
+            if (loc < 0) { err = "internal: update row outside the parent front"; return CS_E_INVALID; }
+            B.rel[B.rel_ptr[s] + a] = loc;
+            B.inv[B.inv_ptr[s] + loc] = a;
+        }
+    }
+    // original entries
+    const int32_t nnz_nat = P.nnz_nat;
+    B.nat_dest.resize(nnz_nat);
+    std::vector<int32_t> olev(nnz_nat);
+    for (int32_t r = 0; r < n; r++)
+        for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e++) {
+            const int32_t kr = B.prow[r], kc = B.pcol[P.nat_col[e]];
+            const int32_t s = sn_of[std::min(kr, kc)];
+            const int32_t li = local(s, kr), lj = local(s, kc);
+            if (li < 0 || lj < 0) { err = "internal: entry outside its front"; return CS_E_INVALID; }
+            B.nat_dest[e] = B.sn_off[s] + (int64_t)lj * B.sn_m[s] + li;
+            olev[e] = B.sn_level[s];
+        }
+    B.olev_ptr.assign(nlev + 1, 0);
+    for (int32_t e = 0; e < nnz_nat; e++) B.olev_ptr[olev[e] + 1]++;
+    for (int32_t l = 0; l < nlev; l++) B.olev_ptr[l + 1] += B.olev_ptr[l];
+    B.olev_pos.resize(nnz_nat);
+    {
+        std::vector<int32_t> fill(B.olev_ptr.begin(), B.olev_ptr.end() - 1);
+        for (int32_t e = 0; e < nnz_nat; e++) B.olev_pos[fill[olev[e]]++] = e;
+    }
+    // natural-position gather lists of the linear and nonlinear stamps
+    auto lists = [&](const std::vector<Contrib>& v, std::vector<int32_t>& ptr, std::vector<int32_t>& ref) {
+        std::vector<int32_t> cnt(nnz_nat + 1, 0);
+        std::vector<int32_t> pos(v.size());
+        for (size_t q = 0; q < v.size(); q++) {
+            const int32_t r = v[q].r;
+            const int32_t* b = P.nat_col.data() + P.nat_rowptr[r];
+            const int32_t* e = P.nat_col.data() + P.nat_rowptr[r + 1];
+            pos[q] = P.nat_rowptr[r] + lower_pos(b, e, v[q].c);
+            cnt[pos[q] + 1]++;
+        }
+        for (int32_t e = 0; e < nnz_nat; e++) cnt[e + 1] += cnt[e];
+        ptr = cnt;
+        ref.assign(v.size(), 0);
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t q = 0; q < v.size(); q++) ref[fill[pos[q]]++] = v[q].ref;
+    };
+    lists(S.G, P.g_ptr, P.g_ref);
+    lists(S.C, P.c_ptr, P.c_ref);
+    lists(S.J, P.jnl_ptr, P.jnl_ref);
+    {
+        std::vector<int32_t> cnt(n + 1, 0);
+        for (auto& rc2 : S.R) cnt[rc2.r + 1]++;
+        for (int32_t r = 0; r < n; r++) cnt[r + 1] += cnt[r];
+        P.rnl_ptr = cnt;
+        P.rnl_ref.assign(S.R.size(), 0);
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (auto& rc2 : S.R) P.rnl_ref[fill[rc2.r]++] = rc2.ref;
+    }
+    std::vector<int32_t> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    set_probes(P, all);
+    return CS_OK;
+}
+
+}  // namespace cedar

@@ -36,7 +36,12 @@ using namespace va_bsimcmg;
 constexpr int kBsimThreads = 256;
 constexpr int kOutPer = 40;
 
-__global__ __launch_bounds__(kBsimThreads) void k_bsimcmg(BsimArgs A) {
+#ifndef CEDAR_BSIM_WAVES
+#define CEDAR_BSIM_WAVES 1      // wavefronts per SIMD the register budget is sized for
+#endif
+
+__global__ __launch_bounds__(kBsimThreads)
+__attribute__((amdgpu_waves_per_eu(CEDAR_BSIM_WAVES, CEDAR_BSIM_WAVES))) void k_bsimcmg(BsimArgs A) {
     __shared__ Card card;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(A.card);   // a Card

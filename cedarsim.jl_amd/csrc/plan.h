@@ -92,6 +92,33 @@ struct Plan {
 
 // Build the plan.  Returns CS_OK or a CS_E_* code with a message in err.
 int build_plan(const cs_netlist* nl, Plan& plan, std::string& err);
+
+// The stages build_plan is made of, shared with the supernodal plan of large
+// circuits (bigplan.h).
+struct Contrib {
+    int32_t r, c, ref;   // ref: sref into the linear or stamp buffer
+};
+struct RowContrib {
+    int32_t r, ref;
+};
+// MNA stamps as contributor lists (G: conductance, C: charge, J: nonlinear
+// Jacobian, R: nonlinear residual) and the natural pattern by rows / columns.
+struct Stamps {
+    std::vector<Contrib> G, C, J;
+    std::vector<RowContrib> R;
+    std::vector<std::vector<int32_t>> rowc, colr;
+};
+// Fills the plan's sizes, parameter layout, linear-stamp recipes, devices,
+// sources, is_diff and natural pattern.
+int stamp_netlist(const cs_netlist* nl, Plan& plan, Stamps& S, std::string& err);
+// Pivot matching: singleton pivots first (prow_of_k / pcol_of_k), then a
+// maximum transversal of the remainder: match_col[c] is the row paired with
+// remaining column c (rem_cols).
+struct PivotMatch {
+    std::vector<int32_t> prow_of_k, pcol_of_k, rem_cols, match_col;
+    int32_t n_singletons = 0;
+};
+int match_pivots(const Plan& plan, const Stamps& S, PivotMatch& M, std::string& err);
 void set_probes(Plan& plan, const std::vector<int32_t>& idx);
 
 // Parameter layout of include/cedarhip.h.

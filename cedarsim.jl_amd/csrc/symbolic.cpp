@@ -67,13 +67,6 @@ void default_params(const cs_netlist* nl, double* p) {
 
 namespace {
 
-struct Contrib {
-    int32_t r, c, ref;   // ref: sref into the linear or stamp buffer
-};
-struct RowContrib {
-    int32_t r, ref;
-};
-
 bool check_node(int32_t v, int32_t n_nodes) { return v >= -1 && v < n_nodes; }
 
 // position of permuted column j inside permuted row i of F, or -1
@@ -132,7 +125,7 @@ void set_probes(Plan& P, const std::vector<int32_t>& idx) {
     for (size_t s = 0; s < idx.size(); s++) P.probe_slot[idx[s]] = (int32_t)s;
 }
 
-int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
+int stamp_netlist(const cs_netlist* nl, Plan& P, Stamps& S, std::string& err) {
     P = Plan();
     if (!nl || nl->n_nodes < 0) { err = "null netlist"; return CS_E_INVALID; }
     const int32_t nn = nl->n_nodes;
@@ -147,8 +140,11 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
     default_params(nl, P.p_default.data());
 
     // ---- 1. stamps ------------------------------------------------------
-    std::vector<Contrib> G, C, J;
-    std::vector<RowContrib> R;
+    S = Stamps();
+    std::vector<Contrib>& G = S.G;
+    std::vector<Contrib>& C = S.C;
+    std::vector<Contrib>& J = S.J;
+    std::vector<RowContrib>& R = S.R;
     auto lin_slot = [&](int32_t kind, int32_t idx) {
         P.lin_kind.push_back(kind);
         P.lin_idx.push_back(idx);
@@ -263,7 +259,8 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
     for (auto& c : C) P.is_diff[c.r] = 1;
 
     // ---- natural pattern -------------------------------------------------
-    std::vector<std::vector<int32_t>> rowc(n);
+    std::vector<std::vector<int32_t>>& rowc = S.rowc;
+    rowc.assign(n, {});
     for (auto* v : {&G, &C, &J})
         for (auto& c : *v) rowc[c.r].push_back(c.c);
     for (auto& r : rowc) {
@@ -275,7 +272,8 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
     P.nat_col.reserve(P.nat_rowptr[n]);
     for (auto& r : rowc) P.nat_col.insert(P.nat_col.end(), r.begin(), r.end());
     P.nnz_nat = P.nat_rowptr[n];
-    std::vector<std::vector<int32_t>> colr(n);
+    std::vector<std::vector<int32_t>>& colr = S.colr;
+    colr.assign(n, {});
     for (int32_t r = 0; r < n; r++)
         for (int32_t c : rowc[r]) colr[c].push_back(r);
     for (int32_t r = 0; r < n; r++)
@@ -283,12 +281,20 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
             err = "structurally singular: unknown " + std::to_string(r) + " has an empty row";
             return CS_E_SINGULAR;
         }
+    return CS_OK;
+}
+
+int match_pivots(const Plan& P, const Stamps& S, PivotMatch& M, std::string& err) {
+    const int32_t n = P.n;
+    const std::vector<std::vector<int32_t>>& rowc = S.rowc;
+    const std::vector<std::vector<int32_t>>& colr = S.colr;
     auto has = [&](int32_t r, int32_t c) {
         return std::binary_search(rowc[r].begin(), rowc[r].end(), c);
     };
-
+    M = PivotMatch();
+    std::vector<int32_t>& prow_of_k = M.prow_of_k;
+    std::vector<int32_t>& pcol_of_k = M.pcol_of_k;
     // ---- 2. pivot order --------------------------------------------------
-    std::vector<int32_t> prow_of_k, pcol_of_k;    // pivot sequence
     std::vector<char> ralive(n, 1), calive(n, 1);
     std::vector<int32_t> rcnt(n), ccnt(n);
     for (int32_t r = 0; r < n; r++) rcnt[r] = (int32_t)rowc[r].size();
@@ -320,7 +326,7 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
             changed = true;
         }
     }
-    P.n_singletons = (int32_t)prow_of_k.size();
+    M.n_singletons = (int32_t)prow_of_k.size();
     for (int32_t r = 0; r < n; r++)
         if (ralive[r] && rcnt[r] == 0) {
             err = "structurally singular at unknown " + std::to_string(r);
@@ -329,10 +335,13 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
 
     // maximum transversal on the remaining block (diagonal first, then
     // augmenting paths)
-    std::vector<int32_t> rem_rows, rem_cols;
+    std::vector<int32_t> rem_rows;
+    std::vector<int32_t>& rem_cols = M.rem_cols;
     for (int32_t r = 0; r < n; r++) if (ralive[r]) rem_rows.push_back(r);
     for (int32_t c = 0; c < n; c++) if (calive[c]) rem_cols.push_back(c);
-    std::vector<int32_t> match_row(n, -1), match_col(n, -1);
+    std::vector<int32_t> match_row(n, -1);
+    std::vector<int32_t>& match_col = M.match_col;
+    match_col.assign(n, -1);
     for (int32_t r : rem_rows)
         if (calive[r] && has(r, r)) { match_row[r] = r; match_col[r] = r; }
     std::vector<int32_t> visit(n, -1);
@@ -377,6 +386,25 @@ int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
             match_row[r] = c;
         }
     }
+    return CS_OK;
+}
+
+int build_plan(const cs_netlist* nl, Plan& P, std::string& err) {
+    Stamps S;
+    int rc = stamp_netlist(nl, P, S, err);
+    if (rc) return rc;
+    PivotMatch M;
+    if ((rc = match_pivots(P, S, M, err))) return rc;
+    P.n_singletons = M.n_singletons;
+    const int32_t n = P.n;
+    const std::vector<Contrib>& G = S.G;
+    const std::vector<Contrib>& C = S.C;
+    const std::vector<Contrib>& J = S.J;
+    const std::vector<RowContrib>& R = S.R;
+    const std::vector<std::vector<int32_t>>& rowc = S.rowc;
+    std::vector<int32_t> prow_of_k = M.prow_of_k, pcol_of_k = M.pcol_of_k;
+    const std::vector<int32_t>& rem_cols = M.rem_cols;
+    const std::vector<int32_t>& match_col = M.match_col;
     // symmetric MMD on the matched remainder
     const int32_t m = (int32_t)rem_cols.size();
     std::vector<int32_t> cidx(n, -1);

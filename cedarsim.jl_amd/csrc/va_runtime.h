@@ -17,6 +17,19 @@
 #define CEDAR_VA_HD
 #endif
 
+// Out-of-line elementary functions and generated analog functions: a compact
+// model calls exp/log/pow and its helper functions at hundreds of sites, and
+// inlining every one makes the device code far larger than the instruction
+// cache.  CEDAR_VA_OUTLINE=0 inlines them again.
+#ifndef CEDAR_VA_OUTLINE
+#define CEDAR_VA_OUTLINE 1
+#endif
+#if CEDAR_VA_OUTLINE
+#define CEDAR_VA_OUT inline __attribute__((noinline))
+#else
+#define CEDAR_VA_OUT inline
+#endif
+
 template <int N>
 struct VaDual {
     double v;
@@ -77,11 +90,15 @@ VA_DUAL_OP(-, a.v - b.v, a.d[i] - b.d[i])
 VA_DUAL_OP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
 #undef VA_DUAL_OP
 
+// Dual quotients take one reciprocal and multiply (value included): an FP64
+// division is a ~10-instruction sequence on the device, and a dual quotient
+// would otherwise pay one per component.  Values stay within an ulp or two
+// of the correctly rounded quotient.
 template <int N>
 CEDAR_VA_HD inline VaDual<N> operator/(const VaDual<N>& a, const VaDual<N>& b) {
     VaDual<N> r;
-    r.v = a.v / b.v;
     const double ib = 1.0 / b.v;
+    r.v = a.v * ib;
     for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
     return r;
 }
@@ -129,15 +146,17 @@ CEDAR_VA_HD inline VaDual<N> operator*(double a, const VaDual<N>& b) { return b 
 template <int N>
 CEDAR_VA_HD inline VaDual<N> operator/(const VaDual<N>& a, double b) {
     VaDual<N> r;
-    r.v = a.v / b;
-    for (int i = 0; i < N; i++) r.d[i] = a.d[i] / b;
+    const double ib = 1.0 / b;
+    r.v = a.v * ib;
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * ib;
     return r;
 }
 template <int N>
 CEDAR_VA_HD inline VaDual<N> operator/(double a, const VaDual<N>& b) {
     VaDual<N> r;
-    r.v = a / b.v;
-    const double k = -r.v / b.v;
+    const double ib = 1.0 / b.v;
+    r.v = a * ib;
+    const double k = -r.v * ib;
     for (int i = 0; i < N; i++) r.d[i] = k * b.d[i];
     return r;
 }
@@ -182,8 +201,9 @@ CEDAR_VA_HD inline VaDual<N> va_chain(const VaDual<N>& x, double fx, double dfx)
     return r;
 }
 
-CEDAR_VA_HD inline double va_exp(double x) { return std::exp(x); }
-CEDAR_VA_HD inline double va_ln(double x) { return x < 0.0 ? NAN : std::log(x); }
+CEDAR_VA_HD CEDAR_VA_OUT double va_exp(double x) { return std::exp(x); }
+CEDAR_VA_HD CEDAR_VA_OUT double va_log(double x) { return std::log(x); }
+CEDAR_VA_HD inline double va_ln(double x) { return x < 0.0 ? NAN : va_log(x); }
 CEDAR_VA_HD inline double va_log10(double x) { return x < 0.0 ? NAN : std::log10(x); }
 CEDAR_VA_HD inline double va_sqrt(double x) { return x < 0.0 ? NAN : std::sqrt(x); }
 CEDAR_VA_HD inline double va_abs(double x) { return std::fabs(x); }
@@ -200,9 +220,9 @@ CEDAR_VA_HD inline double va_tanh(double x) { return std::tanh(x); }
 CEDAR_VA_HD inline double va_asinh(double x) { return std::asinh(x); }
 CEDAR_VA_HD inline double va_floor(double x) { return std::floor(x); }
 CEDAR_VA_HD inline double va_ceil(double x) { return std::ceil(x); }
-CEDAR_VA_HD inline double va_limexp(double x) { return std::exp(x); }
+CEDAR_VA_HD inline double va_limexp(double x) { return va_exp(x); }
 CEDAR_VA_HD inline double va_fmod(double a, double b) { return std::fmod(a, b); }
-CEDAR_VA_HD inline double va_pow(double a, double b) {
+CEDAR_VA_HD CEDAR_VA_OUT double va_pow(double a, double b) {
     // NaNMath.pow: NaN for a negative base with a non-integer exponent
     if (a < 0.0 && std::floor(b) != b) return NAN;
     return std::pow(a, b);
@@ -212,7 +232,7 @@ CEDAR_VA_HD inline double va_hypot(double a, double b) { return std::hypot(a, b)
 
 template <int N>
 CEDAR_VA_HD inline VaDual<N> va_exp(const VaDual<N>& x) {
-    const double e = std::exp(x.v);
+    const double e = va_exp(x.v);
     return va_chain(x, e, e);
 }
 template <int N>
@@ -282,7 +302,7 @@ CEDAR_VA_HD inline VaDual<N> va_pow(const VaDual<N>& a, const VaDual<N>& b) {
                          : (b.v == 1.0 ? 1.0 : ((b.v == 0.0 || b.v > 1.0) ? 0.0 : (double)INFINITY));
     bool db = false;
     for (int i = 0; i < N; i++) db = db || (b.d[i] != 0.0);
-    const double dp = (db && nz) ? y * std::log(std::fabs(a.v)) : 0.0;
+    const double dp = (db && nz) ? y * va_log(std::fabs(a.v)) : 0.0;
     VaDual<N> r;
     r.v = y;
     for (int i = 0; i < N; i++) r.d[i] = b.d[i] != 0.0 ? dp * b.d[i] + dx * a.d[i] : dx * a.d[i];

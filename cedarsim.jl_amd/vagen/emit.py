@@ -400,7 +400,7 @@ class Emitter:
         args = ", ".join((("S " if f.locals.get(n, "real") != "integer" else "int ") + cname(n))
                          for n in f.inputs)
         rt = "S" if f.rtype != "integer" else "int"
-        lines.append(f"template <class S> CEDAR_VA_HD inline {rt} {self.prefix}_fn_{f.name}({args}) {{")
+        lines.append(f"template <class S> CEDAR_VA_HD CEDAR_VA_OUT {rt} {self.prefix}_fn_{f.name}({args}) {{")
         lines.append("    using T = S;")
         for n, t in f.locals.items():
             if n in f.inputs:

@@ -183,6 +183,44 @@ const char* cs_last_error(const cs_ctx* ctx);
 int cs_analyze(const cs_netlist* nl, cs_circuit_info* info, int32_t* lu_rowptr,
                int32_t* lu_col, int32_t* rownat, int32_t* colnat);
 
+/* ---- large circuits: supernodal multifrontal solver -------------------- */
+/* Circuits from CS_SUPERNODAL_MIN_N unknowns up (power grids, meshes) are
+ * factored by a supernodal multifrontal LU: nested-dissection order of the
+ * matched matrix, dense fronts on the device, level by level.  They run on
+ * the plain entry points (cs_tran, cs_dcop, cs_residual, cs_jacobian,
+ * cs_lu_refactor, cs_lu_solve), not in batches.  No counterpart in the
+ * reference, which factors densely in IDA (src/sweeps.jl:456). */
+#define CS_SOLVER_AUTO       0   /* supernodal from CS_SUPERNODAL_MIN_N unknowns */
+#define CS_SOLVER_LEVEL      1   /* entry-level LU schedule (batched kernels)    */
+#define CS_SOLVER_SUPERNODAL 2
+#define CS_SUPERNODAL_MIN_N  8192
+
+typedef struct cs_big_info {
+    int32_t n;            /* unknowns                                      */
+    int32_t n_super;      /* supernodes (fronts)                           */
+    int32_t n_levels;     /* levels of the assembly tree                   */
+    int32_t max_front;    /* largest front order m                         */
+    int32_t max_pivots;   /* most pivots in one front                      */
+    int32_t n_hub;        /* high-degree unknowns ordered last             */
+    int64_t pool;         /* doubles of front storage                      */
+    int64_t nnz_factor;   /* entries of L+U held in the fronts             */
+    double  flops;        /* flops of one factorization                    */
+} cs_big_info;
+
+/* Symbolic analysis of the supernodal solver without a GPU: sizes, and the
+ * pivot order (rownat[k] / colnat[k], may be null). */
+int cs_big_analyze(const cs_netlist* nl, cs_big_info* info, int32_t* rownat, int32_t* colnat);
+
+/* cs_circuit_load with an explicit solver choice (CS_SOLVER_*). */
+int cs_circuit_load_solver(cs_ctx* ctx, const cs_netlist* nl, int32_t solver, int32_t* cid);
+
+/* Sizes of a loaded supernodal circuit (CS_E_INVALID for other circuits). */
+int cs_big_info_get(cs_ctx* ctx, int32_t cid, cs_big_info* info);
+
+/* Device time of the last cs_lu_refactor and cs_lu_solve of a supernodal
+ * circuit, milliseconds. */
+int cs_big_times(cs_ctx* ctx, int32_t cid, double* factor_ms, double* solve_ms);
+
 /* ---- circuits (host symbolic analysis, done once per topology) --------- */
 /* Lowers the netlist, builds the Jacobian pattern, the slot maps for
  * deterministic assembly, the static pivot order and the LU / substitution
