@@ -59,6 +59,11 @@ def main():
     ap.add_argument("--bsim-inst", type=int, default=1 << 20,
                     help="BSIM-CMG device-eval microbenchmark size (0: skip)")
     ap.add_argument("--bsim-only", action="store_true")
+    ap.add_argument("--grid-n", type=int, default=450,
+                    help="power-grid workload: n x n RC mesh (0: skip)")
+    ap.add_argument("--grid-inv", type=int, default=10_000, help="power-grid inverters")
+    ap.add_argument("--grid-tstop", type=float, default=0.2e-9, help="power-grid transient span")
+    ap.add_argument("--grid-only", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only fill and copy a buffer of this many MiB on the device (the known-size "
                          "kernels that scale the HBM byte counters)")
@@ -84,6 +89,11 @@ def main():
     from cedarhip import lib as L
     from cedarhip.workloads import ring_mc, ring_widths
 
+    if args.grid_only:
+        eng = ch.Engine(0)
+        print(json.dumps({"power_grid": grid_section(ch, L, args, eng)}), flush=True)
+        eng.close()
+        return
     if args.bsim_only:
         eng = ch.Engine(0)
         print(json.dumps({"device_eval": bsim_section(ch, args, eng)}), flush=True)
@@ -206,6 +216,8 @@ def main():
         out["adjoint"] = adjoint_section(ch, L, ring_widths, args, eng)
     if args.bsim_inst > 0:
         out["device_eval"] = bsim_section(ch, args, eng)
+    if args.grid_n > 0:
+        out["power_grid"] = grid_section(ch, L, args, eng)
     eng.close()
     print(json.dumps(out), flush=True)
 
@@ -236,6 +248,56 @@ def adjoint_section(ch, L, ring_widths, args, eng):
             "mean_steps": float(res.n_steps.mean()),
             "workload": "51-stage ring, W~U[0.5,4]um (102 widths), supply energy 0-5 ns, "
                         "dE/dW by discrete adjoint, checkpoint every 64 steps"}
+
+
+def grid_section(ch, L, args, eng):
+    """Power-grid workload on the supernodal path: n x n RC mesh (R ~ U[0.5,2]
+    ohm, C ~ U[1,10] fF), inverters on random nodes with PULSE inputs at random
+    phases of a 1 ns period, VDD 1.0 V.  Times the symbolic analysis, the DC
+    operating point, factorization and solve at the operating point (HIP
+    events), and a transient over --grid-tstop."""
+    from cedarhip.workloads import rc_mesh
+    n = args.grid_n
+    f = rc_mesh(n, n, args.grid_inv, seed=args.seed)
+    t0 = time.perf_counter()
+    s = eng.load(f, solver=L.SOLVER_SUPERNODAL)
+    t_load = time.perf_counter() - t0
+    bi = s.big_info
+    log(f"grid: {n}x{n} mesh, {args.grid_inv} inverters, n {s.n}, {bi.n_super} fronts, "
+        f"{bi.n_levels} levels, analysis+upload {t_load:.2f} s")
+    p = f.params()
+    t0 = time.perf_counter()
+    dc = s.dcop(p)
+    t_dc = time.perf_counter() - t0
+    u = np.asarray(dc.u)
+    A = s.jac_matrix(None, u, p, 1e12, 0.0)
+    b = np.random.default_rng(args.seed).standard_normal(s.n)
+    fms, sms = [], []
+    for k in range(3):
+        s.lu_refactor(A.data)
+        s.lu_solve(b)
+        fm, sm = s.lu_times()
+        fms.append(fm)
+        sms.append(sm)
+    fm, sm = min(fms), min(sms)
+    log(f"grid: dc {t_dc:.2f} s ({dc.retcode}), factor {fm:.3f} ms, solve {sm:.3f} ms")
+    o = ch.TranOptions(t_start=0.0, t_stop=args.grid_tstop, reltol=1e-6, abstol=1e-9)
+    t0 = time.perf_counter()
+    sol = s.tran(o, p)
+    wall = time.perf_counter() - t0
+    st = sol.stats
+    log(f"grid: transient 0-{args.grid_tstop * 1e9:g} ns {sol.retcode}: {st.steps} steps, "
+        f"{st.newton_iters} Newton iterations, {wall:.2f} s")
+    return {"value": args.grid_tstop * 1e9 / wall, "unit": "simulated ns/s",
+            "n_unknowns": s.n, "fronts": bi.n_super, "levels": bi.n_levels, "max_front": bi.max_front,
+            "factor_ms": fm, "solve_ms": sm, "factor_flops": bi.flops,
+            "factor_tflops": bi.flops / (fm * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK / 1e12,
+            "factor_bytes_pool": 8 * bi.pool, "dcop_s": t_dc, "analysis_s": t_load,
+            "tran_retcode": sol.retcode, "tran_steps": int(st.steps), "tran_newton_iters": int(st.newton_iters),
+            "tran_wall_s": wall, "ms_per_newton_iter": wall * 1e3 / max(int(st.newton_iters), 1),
+            "workload": f"{n}x{n} RC mesh + {args.grid_inv} square-law inverters (PULSE inputs, random "
+                        f"phase on a 10 ps grid in a 1 ns period), VDD 1.0 V, 0-{args.grid_tstop * 1e9:g} ns, "
+                        "BDF2 reltol 1e-6"}
 
 
 def bsim_section(ch, args, eng):

@@ -23,6 +23,7 @@
 // Everything is bit-reproducible run to run.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -383,6 +384,51 @@ __global__ __launch_bounds__(kBT) void k_mf_gemm(const BigDev* P, double* pool, 
     }
 }
 
+// The same update on the FP64 matrix cores: four wavefronts, each a 32 x 32
+// quarter of the tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A: lane l
+// holds A[l%16][l/16], B: B[l/16][l%16], accumulator element v of lane l is
+// D[l/16 + 4v][l%16], as measured on gfx950).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* pool, const int4* tiles) {
+    __shared__ double A[kNb][kTile + 1];
+    __shared__ double Bt[kNb][kTile + 1];
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
+    const int m = P->sn_m[s], k = sn_k(P, s);
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
+        const int c = q / kTile, i = q % kTile;
+        const bool ok = c < nb;
+        A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
+        Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    dbl4 acc[2][2];
+    for (int bi = 0; bi < 2; bi++)
+        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < nb; kk += 4) {
+        const double a0 = A[kk + q4][wi + r16], a1 = A[kk + q4][wi + 16 + r16];
+        const double b0 = Bt[kk + q4][wj + r16], b1 = Bt[kk + q4][wj + 16 + r16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    for (int bi = 0; bi < 2; bi++)
+        for (int bj = 0; bj < 2; bj++) {
+            const int j = j0 + wj + bj * 16 + r16;
+            if (j >= m) continue;
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + wi + bi * 16 + q4 + 4 * v;
+                if (i < m) G[(int64_t)j * m + i] -= acc[bi][bj][v];
+            }
+        }
+}
+
 // ---- multifrontal solves ----------------------------------------------------------------
 // forward: y = L^-1 b by levels; update vectors of the fronts in uvec
 __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
@@ -546,7 +592,7 @@ struct BigCircuit {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double factor_ms = 0.0, solve_ms = 0.0;
-    bool lin_ready = false;
+    bool mfma = false;   // trailing updates on the matrix cores
 };
 
 namespace {
@@ -724,8 +770,12 @@ void launch_factor(BigCircuit& C, hipStream_t s) {
             hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
             if (pn.trsm_n)
                 hipLaunchKernelGGL(k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_trsm + pn.trsm_b);
-            if (pn.gemm_n)
-                hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+            if (pn.gemm_n) {
+                if (C.mfma)
+                    hipLaunchKernelGGL(k_mf_gemm_mfma, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+                else
+                    hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+            }
         }
     }
 }
@@ -810,6 +860,10 @@ int init_point(BigCircuit& C, hipStream_t s, const double* u0, const cs_tran_opt
 int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::string& err) {
     (void)s;
     auto* C = new BigCircuit();
+    // trailing updates of the large fronts: FP64 matrix cores unless
+    // CEDAR_MF_GEMM=valu asks for the vector-ALU kernel (A/B comparison)
+    const char* g = std::getenv("CEDAR_MF_GEMM");
+    C->mfma = !(g && std::strcmp(g, "valu") == 0);
     int rc = build_bigplan(nl, C->B, err);
     if (rc == CS_OK) rc = upload(*C, err);
     if (rc != CS_OK) {
