@@ -1294,8 +1294,14 @@ int cs_bsimcmg_eval(cs_ctx* ctx, int64_t n, const double* L, const double* nfin,
     a.iters = (int32_t*)d_it;
     const int per_cu = bsim_max_blocks();
     if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "BSIM-CMG kernel does not fit on a CU");
+    // one resident wave of lanes, each walking its instances; every lane keeps
+    // the bias-independent values of its current instance in one column of store
     const int64_t want = (n + 255) / 256;
-    const int nblocks = (int)std::min<int64_t>(want, (int64_t)per_cu * std::max(ctx->n_cu, 1) * 4);
+    const int nblocks = (int)std::min<int64_t>(want, (int64_t)per_cu * std::max(ctx->n_cu, 1));
+    void* d_store = nullptr;
+    a.stride = (int64_t)nblocks * 256;
+    if ((rc = dalloc(&d_store, sizeof(double) * (size_t)bsim_setup_size() * a.stride))) return rc;
+    a.store = (double*)d_store;
     hipEvent_t e0, e1;
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));

@@ -17,24 +17,35 @@ static_assert(kBsimNodes == 6, "BSIM-CMG with internal di, si");
 // and live state grow with the number of partials, and each partial is
 // computed independently, so three two-partial passes give exactly the
 // partials of one six-partial pass at far lower register pressure).
+// The bias-independent part of the model (geometry, binning, temperature)
+// ran once per instance (va_bsimcmg::setup) and left its values in S[k *
+// stride]; each evaluation runs only the bias-dependent rest.
 CEDAR_VA_HD inline __attribute__((noinline)) void eval_pair(const Card& P, const Inst& Ip, double temp,
+                                                            const double* S, long long stride,
                                                             const double* V, int a, int b, D2* I, D2* Q) {
     D2 Vd[kBsimNodes];
     for (int k = 0; k < kBsimNodes; k++) Vd[k] = D2(V[k]);
     Vd[a].d[0] = 1.0;
     Vd[b].d[1] = 1.0;
-    va_bsimcmg::eval<D2>(P, Ip, temp, Vd, I, Q);
+    va_bsimcmg::eval_bias<D2>(P, Ip, temp, S, stride, Vd, I, Q);
+}
+
+// The once-per-instance part; a zero-partial dual keeps the arithmetic of
+// dual-typed temporaries identical to a full evaluation.
+CEDAR_VA_HD inline __attribute__((noinline)) void setup_instance(const Card& P, const Inst& Ip, double temp,
+                                                                 double* S, long long stride) {
+    va_bsimcmg::setup<VaDual<0>>(P, Ip, temp, S, stride);
 }
 
 // One instance: returns the Newton iterations used on the internal nodes (or
 // -1 when they did not converge).
 CEDAR_VA_HD inline int bsimcmg_terminal(const Card& P, const Inst& Ip, double temp,
-                                        const double* vt, double* out) {
+                                        const double* S, long long stride, const double* vt, double* out) {
     double V[kBsimNodes] = {vt[0], vt[1], vt[2], vt[3], vt[0], vt[2]};
     D2 I[kBsimNodes], Q[kBsimNodes];
     int it = 0, conv = 0;
     for (; it < 20; it++) {
-        eval_pair(P, Ip, temp, V, 4, 5, I, Q);
+        eval_pair(P, Ip, temp, S, stride, V, 4, 5, I, Q);
         if (conv) break;
         const double a = I[4].d[0], b = I[4].d[1], c = I[5].d[0], d = I[5].d[1];
         const double f0 = I[4].v, f1 = I[5].v;
@@ -72,7 +83,7 @@ CEDAR_VA_HD inline int bsimcmg_terminal(const Card& P, const Inst& Ip, double te
     // terminal columns, two per pass
     for (int pass = 0; pass < 2; pass++) {
         const int j0 = 2 * pass;
-        eval_pair(P, Ip, temp, V, j0, j0 + 1, I, Q);
+        eval_pair(P, Ip, temp, S, stride, V, j0, j0 + 1, I, Q);
         for (int c = 0; c < 2; c++) {
             const int j = j0 + c;
             const double s0 = -(inv[0][0] * I[4].d[c] + inv[0][1] * I[5].d[c]);

@@ -57,8 +57,10 @@ __attribute__((amdgpu_waves_per_eu(CEDAR_BSIM_WAVES, CEDAR_BSIM_WAVES))) void k_
         ip.NFIN = A.nfin[i];
         ip.given_NFIN = 1;
         inst_defaults(card, ip);
+        double* S = A.store + (blockIdx.x * kBsimThreads + threadIdx.x);
+        setup_instance(card, ip, A.temp, S, A.stride);
         const double vt[4] = {A.v[4 * i], A.v[4 * i + 1], A.v[4 * i + 2], A.v[4 * i + 3]};
-        const int it = bsimcmg_terminal(card, ip, A.temp, vt, A.out + (size_t)kOutPer * i);
+        const int it = bsimcmg_terminal(card, ip, A.temp, S, A.stride, vt, A.out + (size_t)kOutPer * i);
         if (A.iters) A.iters[i] = it;
     }
 }
@@ -68,6 +70,7 @@ __attribute__((amdgpu_waves_per_eu(CEDAR_BSIM_WAVES, CEDAR_BSIM_WAVES))) void k_
 int bsim_card_size() { return (int)sizeof(Card); }
 int bsim_inst_size() { return (int)sizeof(Inst); }
 int bsim_out_per() { return kOutPer; }
+int bsim_setup_size() { return n_setup; }
 
 // Host: defaults for a card whose given parameters are set, and the
 // instance template (given flags only for parameters the caller fixed).

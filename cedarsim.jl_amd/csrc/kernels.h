@@ -102,6 +102,8 @@ struct BsimArgs {
     double temp;            // kelvin
     double* out;            // [n][40]: I[4], Q[4], G[16], C[16]
     int32_t* iters;         // [n] internal-node Newton iterations (may be null)
+    double* store;          // bias-independent values: [bsim_setup_size()][stride], one column per lane
+    int64_t stride;         // = lanes of the launch
 };
 
 constexpr int kSingleThreads = 256;
@@ -145,6 +147,7 @@ int bsim_out_per();
 void bsim_card_defaults(void* card);
 int bsim_set_param(void* card, void* inst, const char* name, double value);
 int bsim_max_blocks();
+int bsim_setup_size();
 hipError_t launch_bsimcmg(const BsimArgs& a, int nblocks, hipStream_t s);
 
 }  // namespace cedar

@@ -213,6 +213,8 @@ class Emitter:
                 return cname(n)
             if n in self.params:
                 return ("Ip." if n in self.inst_set else "P.") + cname(n)
+            if n in getattr(self, "var_subst", {}):
+                return self.var_subst[n]
             return cname(n)
         if isinstance(e, Probe):
             if e.b is None:
@@ -470,6 +472,7 @@ class Emitter:
         for f in self.m.functions.values():
             o.append(self.function(f))
             o.append("")
+        o += self.emit_split()
         # eval
         o.append("// I[k], Q[k]: resistive current and charge leaving node k (KCL order of node_names)")
         o.append("template <class T>")
@@ -485,6 +488,121 @@ class Emitter:
         o.append("}")
         o.append(f"}}  // namespace va_{pf}")
         return "\n".join(o) + "\n"
+
+    # ---- bias-independent prefix ---------------------------------------------------------
+    _BIAS_CALLS = {"ddt", "idt", "ddx", "$abstime", "$realtime", "$limit", "white_noise", "flicker_noise",
+                   "noise_table", "$bound_step", "$discontinuity"}
+
+    def _flatten(self, stmts):
+        out = []
+        for x in stmts:
+            if isinstance(x, Block):
+                out.extend(self._flatten(x.stmts))
+            else:
+                out.append(x)
+        return out
+
+    def _bias_dep(self, x) -> bool:
+        if isinstance(x, (Probe, Contrib)):
+            return True
+        if isinstance(x, Call) and x.name in self._BIAS_CALLS:
+            return True
+        for ch in getattr(x, "__dict__", {}).values():
+            if isinstance(ch, (list, tuple)):
+                for c in ch:
+                    if isinstance(c, (list, tuple)):
+                        if any(self._bias_dep(cc) for cc in c if cc is not None):
+                            return True
+                    elif c is not None and self._bias_dep(c):
+                        return True
+            elif ch is not None and hasattr(ch, "__dict__") and self._bias_dep(ch):
+                return True
+        return False
+
+    def _assigned(self, stmts) -> Set[str]:
+        out: Set[str] = set()
+
+        def walk(x):
+            if isinstance(x, Assign):
+                out.add(x.name)
+            for ch in getattr(x, "__dict__", {}).values():
+                if isinstance(ch, (list, tuple)):
+                    for c in ch:
+                        if isinstance(c, (list, tuple)):
+                            for cc in c:
+                                if cc is not None:
+                                    walk(cc)
+                        elif c is not None:
+                            walk(c)
+                elif ch is not None and hasattr(ch, "__dict__"):
+                    walk(ch)
+        for x in stmts:
+            walk(x)
+        return out
+
+    def emit_split(self) -> List[str]:
+        """setup() runs the statements before the first one that touches the
+        bias (a probe, a contribution, an analog operator) once per instance
+        and stores every variable they assign that the rest reads; eval_bias()
+        runs the rest, reading those values back.  setup + eval_bias compute
+        exactly what eval computes."""
+        flat = self._flatten(list(self.m.analog))
+        split = len(flat)
+        for i, x in enumerate(flat):
+            if self._bias_dep(x):
+                split = i
+                break
+        prefix, suffix = flat[:split], flat[split:]
+        read_suffix: Set[str] = set()
+        for x in suffix:
+            read_suffix |= refs(x)
+        a_pre = self._assigned(prefix)
+        a_suf = self._assigned(suffix)
+        saved = [n for n in self.m.vars if n in a_pre and n in read_suffix]
+        slot = {n: k for k, n in enumerate(saved)}
+        o: List[str] = []
+        o.append(f"// bias-independent part of eval: {len(prefix)} of {len(flat)} top-level statements, "
+                 f"{len(saved)} values kept per instance")
+        o.append(f"constexpr int n_setup = {len(saved)};")
+        o.append("// S[k * stride]: saved value k of this instance")
+        o.append("template <class T = double>")
+        o.append("CEDAR_VA_HD inline void setup(const Card& P, const Inst& Ip, double temp, double* S, long long stride) {")
+        for n, t in self.m.vars.items():
+            ct = "int" if t == "integer" else ("T" if n in self.dual else "double")
+            o.append(f"    {ct} {cname(n)} = 0;")
+        body: List[str] = []
+        for x in prefix:
+            self.st(x, "    ", body)
+        o += body
+        for n in saved:
+            o.append(f"    S[{slot[n]} * stride] = (double)va_val({cname(n)});")
+        o.append("}")
+        o.append("template <class T>")
+        o.append("CEDAR_VA_HD inline void eval_bias(const Card& P, const Inst& Ip, double temp, "
+                 "const double* __restrict__ S, long long stride, const T* V, T* I, T* Q) {")
+        o.append("    for (int k = 0; k < n_nodes; k++) { I[k] = T(0.0); Q[k] = T(0.0); }")
+        subst = {}
+        for n, t in self.m.vars.items():
+            ct = "int" if t == "integer" else ("T" if n in self.dual else "double")
+            if n in slot and n not in a_suf:
+                # read-only here: read from the per-instance store at each use
+                ld = f"S[{slot[n]} * stride]"
+                subst[n] = f"((int){ld})" if t == "integer" else (f"T({ld})" if n in self.dual else ld)
+                continue
+            init = "0"
+            if n in slot:
+                init = f"(int)S[{slot[n]} * stride]" if t == "integer" else \
+                    (f"T(S[{slot[n]} * stride])" if n in self.dual else f"S[{slot[n]} * stride]")
+            o.append(f"    {ct} {cname(n)} = {init};")
+        self.var_subst = subst
+        body = []
+        for x in suffix:
+            self.st(x, "    ", body)
+        self.var_subst = {}
+        o += body
+        o.append("}")
+        o.append("")
+        return o
 
     def check_default_refs(self, p, allow_inst: bool):
         def walk(e):
