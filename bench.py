@@ -32,6 +32,13 @@ import numpy as np  # noqa: E402
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 
+# HBM bytes per launch of the packed transient kernel on the default workload
+# (4096 rings, 0-10 ns, seed 2024), FETCH_SIZE + WRITE_SIZE from two
+# rocprofv3 --pmc passes, each scaled by 1.00 (a 1 GiB reduction reads 1.000
+# GiB on FETCH_SIZE, a 1 GiB fill writes 1.000 GiB on WRITE_SIZE):
+# profiles/k_tran_pack_ring_mc_4096x10ns_traffic.md
+MEASURED_TRAFFIC = {(4096, 10e-9, 2024, "lds"): (3.24428e8 + 3.08742e9) * 1024}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -172,7 +179,8 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.n_inst, args.t_stop, args.seed,
+                                             "lds" if batch.kernel == L.BATCH_KERNEL_LDS else "global")),
             "kernel": "k_tran_pack" if batch.kernel == L.BATCH_KERNEL_LDS else "k_tran_batch",
             "kernel_ms": kernel_ms,
             "alg_bytes_per_launch": res.alg_bytes,

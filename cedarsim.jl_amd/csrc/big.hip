@@ -593,6 +593,7 @@ struct BigCircuit {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double factor_ms = 0.0, solve_ms = 0.0;
     bool mfma = false;   // trailing updates on the matrix cores
+    bool use_graph = true;
 };
 
 namespace {
@@ -790,12 +791,10 @@ void launch_solve(BigCircuit& C, hipStream_t s) {
                            C.d_fronts + it->fr_b);
 }
 
-int ensure_graph(BigCircuit& C, hipStream_t s, std::string& err) {
-    if (C.exec) return CS_OK;
+void launch_iteration(BigCircuit& C, hipStream_t s) {
     const BigDev* P = C.d_dev;
     const int gr = grid_for(C.B.n);
     const int ge = grid_for(C.B.base.n_mos + C.B.base.n_src);
-    BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_eval, dim3(ge), dim3(kBT), 0, s, P, C.p, C.u, C.sp, C.st);
     hipLaunchKernelGGL(k_assemble, dim3(gr), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, C.qx, C.st, C.u0,
                        C.Aval, C.x, C.part);
@@ -805,6 +804,21 @@ int ensure_graph(BigCircuit& C, hipStream_t s, std::string& err) {
     hipLaunchKernelGGL(k_update, dim3(gr), dim3(kBT), 0, s, P, C.sp, C.x, C.u, C.norm, C.part);
     hipLaunchKernelGGL(k_upd_reduce, dim3(1), dim3(kBT), 0, s, P, C.sp, C.part, gr, C.norm);
     (void)hipMemcpyAsync(C.h_norm, C.norm, NRM_SLOTS * sizeof(double), hipMemcpyDeviceToHost, s);
+}
+
+int run_iter(BigCircuit& C, hipStream_t s, std::string& err) {
+    if (C.exec) BCHK(hipGraphLaunch(C.exec, s));
+    else launch_iteration(C, s);
+    BCHK(hipStreamSynchronize(s));
+    return CS_OK;
+}
+
+// The Newton iteration as one hipGraph (CEDAR_MF_GRAPH=0: plain stream
+// launches, for A/B timing).
+int ensure_graph(BigCircuit& C, hipStream_t s, std::string& err) {
+    if (C.exec || !C.use_graph) return CS_OK;
+    BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_iteration(C, s);
     BCHK(hipStreamEndCapture(s, &C.graph));
     BCHK(hipGraphInstantiate(&C.exec, C.graph, nullptr, nullptr, 0));
     return CS_OK;
@@ -819,11 +833,6 @@ int set_params(BigCircuit& C, hipStream_t s, const double* p, std::string& err) 
     return CS_OK;
 }
 
-int run_iter(BigCircuit& C, hipStream_t s, std::string& err) {
-    BCHK(hipGraphLaunch(C.exec, s));
-    BCHK(hipStreamSynchronize(s));
-    return CS_OK;
-}
 
 int set_step(BigCircuit& C, hipStream_t s, const StepDev& sp, std::string& err) {
     BCHK(hipMemcpyAsync(C.sp, &sp, sizeof(StepDev), hipMemcpyHostToDevice, s));
@@ -864,6 +873,8 @@ int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::strin
     // CEDAR_MF_GEMM=valu asks for the vector-ALU kernel (A/B comparison)
     const char* g = std::getenv("CEDAR_MF_GEMM");
     C->mfma = !(g && std::strcmp(g, "valu") == 0);
+    const char* gg = std::getenv("CEDAR_MF_GRAPH");
+    C->use_graph = !(gg && std::strcmp(gg, "0") == 0);
     int rc = build_bigplan(nl, C->B, err);
     if (rc == CS_OK) rc = upload(*C, err);
     if (rc != CS_OK) {
