@@ -49,6 +49,9 @@ struct BigDev {
     const int32_t *jnl_ptr, *jnl_ref, *rnl_ptr, *rnl_ref, *is_diff, *prow, *pcol;
     const int4 *mos, *src;
     const int32_t *sn_first, *sn_m, *sn_rptr, *sn_rows, *ch_ptr, *ch_sn, *rel_ptr, *rel, *inv, *olev_pos;
+    const int32_t *osn_ptr, *osn_pos;
+    const int32_t *row_long, *long_rows;   // rows too long for one lane (supply node, ...)
+    int32_t n_long, pad_;
     const int64_t *sn_off, *inv_ptr, *nat_dest;
 };
 
@@ -136,6 +139,7 @@ __global__ __launch_bounds__(kBT) void k_assemble(const BigDev* P, const double*
     const bool use_pin = sp.kind == STEP_UIC && pin;
     double lmax = 0.0;
     for (int r = blockIdx.x * kBT + threadIdx.x; r < P->n; r += gridDim.x * kBT) {
+        if (P->row_long[r]) continue;
         const int e0 = P->rowptr[r], e1 = P->rowptr[r + 1];
         double res;
         if (use_pin && P->is_diff[r]) {
@@ -160,6 +164,44 @@ __global__ __launch_bounds__(kBT) void k_assemble(const BigDev* P, const double*
     }
     const double m = blk_reduce(lmax, red, Max());
     if (threadIdx.x == 0 && part) part[blockIdx.x] = m;
+}
+
+// A row with thousands of entries (a supply node touching every PMOS): one
+// workgroup per row, fixed-order block sums.
+__global__ __launch_bounds__(kBT) void k_assemble_long(const BigDev* P, const double* Glin, const double* Clin,
+                                                       const StepDev* spp, const double* u, const double* qx,
+                                                       const double* st, const double* pin, double* Aval,
+                                                       double* x, double* part) {
+    __shared__ double red[kBT];
+    const StepDev sp = *spp;
+    const double a0 = sp.kind == STEP_TRAN ? sp.a0 : 0.0;
+    const bool use_q = sp.kind == STEP_TRAN && qx;
+    const bool use_pin = sp.kind == STEP_UIC && pin;
+    const int r = P->long_rows[blockIdx.x];
+    const int e0 = P->rowptr[r], e1 = P->rowptr[r + 1];
+    double res;
+    if (use_pin && P->is_diff[r]) {
+        for (int e = e0 + threadIdx.x; e < e1; e += kBT) Aval[e] = P->col[e] == r ? 1.0 : 0.0;
+        res = u[r] - pin[r];
+    } else {
+        double acc = 0.0;
+        for (int e = e0 + threadIdx.x; e < e1; e += kBT) {
+            const double jl = Glin[e] + a0 * Clin[e];
+            acc += jl * u[P->col[e]];
+            double v = jl;
+            for (int q = P->jnl_ptr[e]; q < P->jnl_ptr[e + 1]; q++) v += sref_v(st, P->jnl_ref[q]);
+            Aval[e] = v;
+        }
+        double nl = 0.0;
+        for (int q = P->rnl_ptr[r] + threadIdx.x; q < P->rnl_ptr[r + 1]; q += kBT) nl += sref_v(st, P->rnl_ref[q]);
+        const double sa = blk_reduce(acc, red, Sum());
+        const double sn = blk_reduce(nl, red, Sum());
+        res = (use_q ? sa + qx[r] : sa) + sn;
+    }
+    if (threadIdx.x == 0) {
+        if (x) x[P->prow[r]] = -res;
+        if (part) part[blockIdx.x] = fabs(res);
+    }
 }
 
 __global__ __launch_bounds__(kBT) void k_res_reduce(const double* part, int nb, double* norm) {
@@ -429,6 +471,185 @@ __global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* p
         }
 }
 
+// ---- fused per-level front kernels (one workgroup per front) ------------------------------
+// Small fronts: children's contribution blocks and the front's Jacobian
+// entries are gathered straight into LDS, factored there and written once.
+__global__ __launch_bounds__(kBT) void k_mf_lvl_small(const BigDev* P, double* pool, double* dinv,
+                                                      const double* Aval, const int32_t* fronts) {
+    extern __shared__ double lds_f[];
+    double* F = lds_f;
+    const int s = fronts[blockIdx.x];
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int64_t off = P->sn_off[s];
+    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
+    for (int q = threadIdx.x; q < m * m; q += kBT) {
+        const int j = q / m, i = q % m;
+        double v = 0.0;
+        for (int cq = c0; cq < c1; cq++) {
+            const int c = P->ch_sn[cq];
+            const int* iv = P->inv + P->inv_ptr[c];
+            const int a = iv[i], b = iv[j];
+            if (a >= 0 && b >= 0) {
+                const int mc = P->sn_m[c], kc = sn_k(P, c);
+                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
+            }
+        }
+        F[q] = v;
+    }
+    __syncthreads();
+    for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += kBT) {
+        const int e = P->osn_pos[q];
+        F[P->nat_dest[e] - off] += Aval[e];
+    }
+    __syncthreads();
+    for (int p = 0; p < k; p++) {
+        const double ip = 1.0 / F[p * m + p];
+        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) F[p * m + i] *= ip;
+        if (threadIdx.x == 0) dinv[f0 + p] = ip;
+        __syncthreads();
+        const int w = m - p - 1;
+        for (int q = threadIdx.x; q < w * w; q += kBT) {
+            const int j = p + 1 + q / w, i = p + 1 + q % w;
+            F[j * m + i] -= F[p * m + i] * F[j * m + p];
+        }
+        __syncthreads();
+    }
+    double* G = pool + off;
+    for (int q = threadIdx.x; q < m * m; q += kBT) G[q] = F[q];
+}
+
+// Global-memory phases of one workgroup: stores complete and the CU's L1 is
+// invalidated before any wavefront reads what another one wrote.
+__device__ __forceinline__ void gsync() {
+    __threadfence();
+    __syncthreads();
+}
+
+// Large fronts, one workgroup each: gather, then blocked right-looking
+// partial LU in global memory (L2-resident): diagonal block in LDS, row and
+// column triangular solves, trailing update by 64 x 64 tiles on the FP64
+// matrix cores.
+__global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* pool, double* dinv,
+                                                    const double* Aval, const int32_t* fronts) {
+    __shared__ double D[kNb * kNb];
+    __shared__ double invd[kNb];
+    __shared__ double A[kNb][kTile + 1];
+    __shared__ double Bt[kNb][kTile + 1];
+    const int s = fronts[blockIdx.x];
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int64_t off = P->sn_off[s];
+    double* G = pool + off;
+    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
+    for (int q = threadIdx.x; q < m * m; q += kBT) {
+        const int j = q / m, i = q % m;
+        double v = 0.0;
+        for (int cq = c0; cq < c1; cq++) {
+            const int c = P->ch_sn[cq];
+            const int* iv = P->inv + P->inv_ptr[c];
+            const int a = iv[i], b = iv[j];
+            if (a >= 0 && b >= 0) {
+                const int mc = P->sn_m[c], kc = sn_k(P, c);
+                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
+            }
+        }
+        G[q] = v;
+    }
+    gsync();
+    for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += kBT) {
+        const int e = P->osn_pos[q];
+        G[P->nat_dest[e] - off] += Aval[e];
+    }
+    gsync();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    for (int p0 = 0; p0 < k; p0 += kNb) {
+        const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+        const int p1 = p0 + nb;
+        // diagonal block
+        for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+            const int j = q / nb, i = q % nb;
+            D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
+        }
+        __syncthreads();
+        for (int p = 0; p < nb; p++) {
+            const double ip = 1.0 / D[p * kNb + p];
+            for (int i = p + 1 + threadIdx.x; i < nb; i += kBT) D[p * kNb + i] *= ip;
+            if (threadIdx.x == 0) {
+                dinv[f0 + p0 + p] = ip;
+                invd[p] = ip;
+            }
+            __syncthreads();
+            const int w = nb - p - 1;
+            for (int q = threadIdx.x; q < w * w; q += kBT) {
+                const int j = p + 1 + q / w, i = p + 1 + q % w;
+                D[j * kNb + i] -= D[p * kNb + i] * D[j * kNb + p];
+            }
+            __syncthreads();
+        }
+        for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+            const int j = q / nb, i = q % nb;
+            G[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
+        }
+        // L21 rows and U12 columns (the diagonal block from LDS)
+        for (int r = p1 + threadIdx.x; r < m; r += kBT) {
+            double x[kNb];
+            for (int c = 0; c < nb; c++) x[c] = G[(int64_t)(p0 + c) * m + r];
+            for (int c = 0; c < nb; c++) {
+                double v = x[c];
+                for (int c2 = 0; c2 < c; c2++) v -= x[c2] * D[c * kNb + c2];
+                x[c] = v * invd[c];
+            }
+            for (int c = 0; c < nb; c++) G[(int64_t)(p0 + c) * m + r] = x[c];
+        }
+        for (int r = p1 + threadIdx.x; r < m; r += kBT) {
+            double* col = G + (int64_t)r * m + p0;
+            double x[kNb];
+            for (int c = 0; c < nb; c++) x[c] = col[c];
+            for (int c = 0; c < nb; c++) {
+                double v = x[c];
+                for (int c2 = 0; c2 < c; c2++) v -= D[c2 * kNb + c] * x[c2];
+                x[c] = v;
+            }
+            for (int c = 0; c < nb; c++) col[c] = x[c];
+        }
+        gsync();
+        // trailing update
+        for (int j0 = p1; j0 < m; j0 += kTile)
+            for (int i0 = p1; i0 < m; i0 += kTile) {
+                for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
+                    const int c = q / kTile, i = q % kTile;
+                    const bool ok = c < nb;
+                    A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
+                    Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
+                }
+                __syncthreads();
+                dbl4 acc[2][2];
+                for (int bi = 0; bi < 2; bi++)
+                    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = dbl4{0.0, 0.0, 0.0, 0.0};
+                for (int kk = 0; kk < nb; kk += 4) {
+                    const double a0 = A[kk + q4][wi + r16], a1 = A[kk + q4][wi + 16 + r16];
+                    const double b0 = Bt[kk + q4][wj + r16], b1 = Bt[kk + q4][wj + 16 + r16];
+                    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                }
+                for (int bi = 0; bi < 2; bi++)
+                    for (int bj = 0; bj < 2; bj++) {
+                        const int j = j0 + wj + bj * 16 + r16;
+                        if (j >= m) continue;
+                        for (int v = 0; v < 4; v++) {
+                            const int i = i0 + wi + bi * 16 + q4 + 4 * v;
+                            if (i < m) G[(int64_t)j * m + i] -= acc[bi][bj][v];
+                        }
+                    }
+                __syncthreads();
+            }
+        gsync();
+    }
+}
+
 // ---- multifrontal solves ----------------------------------------------------------------
 // forward: y = L^-1 b by levels; update vectors of the fronts in uvec
 __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
@@ -501,9 +722,20 @@ __global__ __launch_bounds__(kBT) void k_predict(int n, int np, int k, double w0
     }
 }
 
+__global__ __launch_bounds__(kBT) void k_charges_long(const BigDev* P, const double* Clin, const double* v,
+                                                      double* q) {
+    __shared__ double red[kBT];
+    const int r = P->long_rows[blockIdx.x];
+    double acc = 0.0;
+    for (int e = P->rowptr[r] + threadIdx.x; e < P->rowptr[r + 1]; e += kBT) acc += Clin[e] * v[P->col[e]];
+    const double t = blk_reduce(acc, red, Sum());
+    if (threadIdx.x == 0) q[r] = t;
+}
+
 __global__ __launch_bounds__(kBT) void k_charges(const BigDev* P, const double* Clin, const double* v,
                                                  double* q) {
     for (int r = blockIdx.x * kBT + threadIdx.x; r < P->n; r += gridDim.x * kBT) {
+        if (P->row_long[r]) continue;
         double acc = 0.0;
         for (int e = P->rowptr[r]; e < P->rowptr[r + 1]; e++) acc += Clin[e] * v[P->col[e]];
         q[r] = acc;
@@ -577,7 +809,7 @@ struct BigCircuit {
         int diag_b, diag_n, trsm_b, trsm_n, gemm_b, gemm_n;
     };
     struct Level {
-        int tile_b, tile_n, small_b, small_n, fr_b, fr_n, orig_b, orig_e;
+        int tile_b, tile_n, small_b, small_n, fr_b, fr_n, orig_b, orig_e, big_b, big_n;
         size_t small_lds, solve_lds;
         std::vector<Panel> panels;
     };
@@ -594,6 +826,7 @@ struct BigCircuit {
     double factor_ms = 0.0, solve_ms = 0.0;
     bool mfma = false;   // trailing updates on the matrix cores
     bool use_graph = true;
+    bool fused = false;  // one workgroup per front and level (else multi-launch panels)
 };
 
 namespace {
@@ -620,6 +853,12 @@ int upload(BigCircuit& C, std::string& err) {
     const Plan& P = B.base;
     std::vector<char> by;
     std::vector<int4> mos4, src4;
+    std::vector<int32_t> row_long(B.n, 0), long_rows;
+    for (int32_t r = 0; r < B.n; r++) {
+        int64_t w = P.nat_rowptr[r + 1] - P.nat_rowptr[r] + (P.rnl_ptr[r + 1] - P.rnl_ptr[r]);
+        for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e++) w += P.jnl_ptr[e + 1] - P.jnl_ptr[e];
+        if (w > 256) { row_long[r] = 1; long_rows.push_back(r); }
+    }
     for (int32_t q = 0; q < P.n_mos; q++)
         mos4.push_back(make_int4(P.mos[4 * q], P.mos[4 * q + 1], P.mos[4 * q + 2], P.mos[4 * q + 3]));
     for (auto& s : P.src) src4.push_back(make_int4(s.kind, s.off, s.len, s.dc));
@@ -633,7 +872,8 @@ int upload(BigCircuit& C, std::string& err) {
            o_chp = pack_add(by, B.ch_ptr), o_chs = pack_add(by, B.ch_sn), o_relp = pack_add(by, B.rel_ptr),
            o_rel = pack_add(by, B.rel), o_inv = pack_add(by, B.inv), o_olp = pack_add(by, B.olev_pos),
            o_soff = pack_add(by, B.sn_off), o_invp = pack_add(by, B.inv_ptr), o_nd = pack_add(by, B.nat_dest),
-           o_probe = pack_add(by, P.probe);
+           o_probe = pack_add(by, P.probe), o_osp = pack_add(by, B.osn_ptr), o_osq = pack_add(by, B.osn_pos),
+           o_rl = pack_add(by, row_long), o_lr = pack_add(by, long_rows);
     BCHK(hipMalloc(&C.dbuf, by.size()));
     BCHK(hipMemcpy(C.dbuf, by.data(), by.size(), hipMemcpyHostToDevice));
     char* b = (char*)C.dbuf;
@@ -652,6 +892,8 @@ int upload(BigCircuit& C, std::string& err) {
     d.sn_first = I(o_sf); d.sn_m = I(o_sm); d.sn_rptr = I(o_srp); d.sn_rows = I(o_srow);
     d.ch_ptr = I(o_chp); d.ch_sn = I(o_chs); d.rel_ptr = I(o_relp); d.rel = I(o_rel); d.inv = I(o_inv);
     d.olev_pos = I(o_olp); d.sn_off = L(o_soff); d.inv_ptr = L(o_invp); d.nat_dest = L(o_nd);
+    d.osn_ptr = I(o_osp); d.osn_pos = I(o_osq);
+    d.row_long = I(o_rl); d.long_rows = I(o_lr); d.n_long = (int32_t)long_rows.size();
     C.d_probe = (int32_t*)(b + o_probe);
     BCHK(hipMalloc(&C.d_dev, sizeof(BigDev)));
     BCHK(hipMemcpy(C.d_dev, &C.hd, sizeof(BigDev), hipMemcpyHostToDevice));
@@ -683,6 +925,9 @@ int upload(BigCircuit& C, std::string& err) {
         Lv.fr_b = (int)fronts.size();
         for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) fronts.push_back(B.lev_sn[q]);
         Lv.fr_n = (int)fronts.size() - Lv.fr_b;
+        Lv.big_b = (int)fronts.size();
+        fronts.insert(fronts.end(), large.begin(), large.end());
+        Lv.big_n = (int)large.size();
         Lv.solve_lds = sizeof(double) * (size_t)std::max(m_max, 1);
         Lv.orig_b = B.olev_ptr[l];
         Lv.orig_e = B.olev_ptr[l + 1];
@@ -726,7 +971,8 @@ int upload(BigCircuit& C, std::string& err) {
     // workspace
     const size_t n = B.n, nnz = P.nnz_nat;
     const size_t nd = P.lay.n_params + P.n_lin + 3 * nnz + (size_t)B.pool + 16 * n + P.n_stamp +
-                      B.rel_ptr[B.ns] + 2 * kRowBlocks + NRM_SLOTS + 8 + sizeof(StepDev) / 8 + 64;
+                      B.rel_ptr[B.ns] + 2 * kRowBlocks + (size_t)C.hd.n_long + 16 + NRM_SLOTS + 8 +
+                      sizeof(StepDev) / 8 + 64;
     BCHK(hipMalloc(&C.wbuf, nd * sizeof(double)));
     BCHK(hipMemset(C.wbuf, 0, nd * sizeof(double)));
     double* q = (double*)C.wbuf;
@@ -748,7 +994,7 @@ int upload(BigCircuit& C, std::string& err) {
     for (auto& h : C.Q) h = take(n);
     C.st = take(std::max<int32_t>(P.n_stamp, 1));
     C.uvec = take(std::max<int32_t>(B.rel_ptr[B.ns], 1));
-    C.part = take(2 * kRowBlocks);
+    C.part = take(2 * kRowBlocks + (size_t)C.hd.n_long + 16);
     C.norm = take(NRM_SLOTS);
     C.scal = take(4);
     C.sp = (StepDev*)take(sizeof(StepDev) / 8 + 1);
@@ -759,6 +1005,17 @@ int upload(BigCircuit& C, std::string& err) {
 
 void launch_factor(BigCircuit& C, hipStream_t s) {
     const BigDev* P = C.d_dev;
+    if (C.fused) {
+        for (auto& Lv : C.levels) {
+            if (Lv.small_n)
+                hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
+                                   C.Aval, C.d_fronts + Lv.small_b);
+            if (Lv.big_n)
+                hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
+                                   C.d_fronts + Lv.big_b);
+        }
+        return;
+    }
     for (auto& Lv : C.levels) {
         if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
         if (Lv.orig_e > Lv.orig_b)
@@ -791,14 +1048,31 @@ void launch_solve(BigCircuit& C, hipStream_t s) {
                            C.d_fronts + it->fr_b);
 }
 
+// Jacobian and residual: short rows one lane each, long rows one workgroup
+// each; per-block residual maxima in part[0, gr + n_long)
+int launch_assemble(BigCircuit& C, hipStream_t s, const double* qx, const double* pin, double* x, double* part) {
+    const BigDev* P = C.d_dev;
+    const int gr = grid_for(C.B.n);
+    hipLaunchKernelGGL(k_assemble, dim3(gr), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, qx, C.st, pin,
+                       C.Aval, x, part);
+    if (C.hd.n_long)
+        hipLaunchKernelGGL(k_assemble_long, dim3(C.hd.n_long), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, qx,
+                           C.st, pin, C.Aval, x, part ? part + gr : nullptr);
+    return gr + C.hd.n_long;
+}
+
+void launch_charges(BigCircuit& C, hipStream_t s, const double* v, double* q) {
+    hipLaunchKernelGGL(k_charges, dim3(grid_for(C.B.n)), dim3(kBT), 0, s, C.d_dev, C.Clin, v, q);
+    if (C.hd.n_long) hipLaunchKernelGGL(k_charges_long, dim3(C.hd.n_long), dim3(kBT), 0, s, C.d_dev, C.Clin, v, q);
+}
+
 void launch_iteration(BigCircuit& C, hipStream_t s) {
     const BigDev* P = C.d_dev;
     const int gr = grid_for(C.B.n);
     const int ge = grid_for(C.B.base.n_mos + C.B.base.n_src);
     hipLaunchKernelGGL(k_eval, dim3(ge), dim3(kBT), 0, s, P, C.p, C.u, C.sp, C.st);
-    hipLaunchKernelGGL(k_assemble, dim3(gr), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, C.qx, C.st, C.u0,
-                       C.Aval, C.x, C.part);
-    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(kBT), 0, s, C.part, gr, C.norm);
+    const int nb = launch_assemble(C, s, C.qx, C.u0, C.x, C.part);
+    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(kBT), 0, s, C.part, nb, C.norm);
     launch_factor(C, s);
     launch_solve(C, s);
     hipLaunchKernelGGL(k_update, dim3(gr), dim3(kBT), 0, s, P, C.sp, C.x, C.u, C.norm, C.part);
@@ -873,6 +1147,8 @@ int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::strin
     // CEDAR_MF_GEMM=valu asks for the vector-ALU kernel (A/B comparison)
     const char* g = std::getenv("CEDAR_MF_GEMM");
     C->mfma = !(g && std::strcmp(g, "valu") == 0);
+    const char* fm = std::getenv("CEDAR_MF_FRONT");
+    C->fused = fm && std::strcmp(fm, "fused") == 0;
     const char* gg = std::getenv("CEDAR_MF_GRAPH");
     C->use_graph = !(gg && std::strcmp(gg, "0") == 0);
     int rc = build_bigplan(nl, C->B, err);
@@ -963,7 +1239,7 @@ int big_tran(BigCircuit* C, hipStream_t s, const double* p, const double* u0, co
     int kout = 0;
     if (rc == CS_OK) {
         BCHK(hipMemcpyAsync(C->H[0], C->u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_charges, dim3(gr), dim3(kBT), 0, s, C->d_dev, C->Clin, C->u, C->Q[0]);
+        launch_charges(*C, s, C->u, C->Q[0]);
         double T0 = o->t_start, T1 = 0.0, T2 = 0.0;
         int npts = 1;
         double h = sd.dt_init;
@@ -1014,7 +1290,7 @@ int big_tran(BigCircuit* C, hipStream_t s, const double* p, const double* u0, co
                 if (h < sd.dt_min) { rc = CS_E_DTMIN; break; }
                 continue;
             }
-            hipLaunchKernelGGL(k_charges, dim3(gr), dim3(kBT), 0, s, C->d_dev, C->Clin, C->u, C->qn);
+            launch_charges(*C, s, C->u, C->qn);
             double fac = 2.0;
             if (npts >= 2) {
                 const double c = lte_coeff(k, tn, T0, T1, T2, a0, a1, a2);
@@ -1101,13 +1377,12 @@ int big_residual(BigCircuit* C, hipStream_t s, double* out, const double* du, co
         // the charge term d/dt q(u) = C du enters as qx with a0 = 0
         if (du) BCHK(hipMemcpyAsync(C->up, du, sizeof(double) * n, hipMemcpyHostToDevice, s));
         else BCHK(hipMemsetAsync(C->up, 0, sizeof(double) * n, s));
-        hipLaunchKernelGGL(k_charges, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Clin, C->up, C->qx);
+        launch_charges(*C, s, C->up, C->qx);
     }
     if ((rc = set_step(*C, s, sp, err))) return rc;
     hipLaunchKernelGGL(k_eval, dim3(grid_for(C->B.base.n_mos + C->B.base.n_src)), dim3(kBT), 0, s, C->d_dev, C->p,
                        C->u, C->sp, C->st);
-    hipLaunchKernelGGL(k_assemble, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Glin, C->Clin, C->sp, C->u,
-                       C->qx, C->st, nullptr, C->Aval, C->x, C->part);
+    launch_assemble(*C, s, C->qx, nullptr, C->x, C->part);
     // x[prow[r]] = -F(r)
     hipLaunchKernelGGL(k_perm_out, dim3(grid_for(n)), dim3(kBT), 0, s, n, C->hd.prow, C->x, C->up);
     std::vector<double> h(n);
@@ -1130,8 +1405,7 @@ int big_jacobian(BigCircuit* C, hipStream_t s, double* vals, const double* du, c
     if ((rc = set_step(*C, s, sp, err))) return rc;
     hipLaunchKernelGGL(k_eval, dim3(grid_for(C->B.base.n_mos + C->B.base.n_src)), dim3(kBT), 0, s, C->d_dev, C->p,
                        C->u, C->sp, C->st);
-    hipLaunchKernelGGL(k_assemble, dim3(grid_for(n)), dim3(kBT), 0, s, C->d_dev, C->Glin, C->Clin, C->sp, C->u,
-                       nullptr, C->st, nullptr, C->Aval, nullptr, nullptr);
+    launch_assemble(*C, s, nullptr, nullptr, nullptr, nullptr);
     BCHK(hipMemcpyAsync(vals, C->Aval, sizeof(double) * C->B.base.nnz_nat, hipMemcpyDeviceToHost, s));
     BCHK(hipStreamSynchronize(s));
     return CS_OK;

@@ -424,6 +424,18 @@ int build_bigplan(const cs_netlist* nl, BigPlan& B, std::string& err) {
             B.nat_dest[e] = B.sn_off[s] + (int64_t)lj * B.sn_m[s] + li;
             olev[e] = B.sn_level[s];
         }
+    {
+        std::vector<int32_t> osn(nnz_nat);
+        for (int32_t r = 0; r < n; r++)
+            for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e++)
+                osn[e] = sn_of[std::min(B.prow[r], B.pcol[P.nat_col[e]])];
+        B.osn_ptr.assign(ns + 1, 0);
+        for (int32_t e = 0; e < nnz_nat; e++) B.osn_ptr[osn[e] + 1]++;
+        for (int32_t q = 0; q < ns; q++) B.osn_ptr[q + 1] += B.osn_ptr[q];
+        B.osn_pos.resize(nnz_nat);
+        std::vector<int32_t> fill(B.osn_ptr.begin(), B.osn_ptr.end() - 1);
+        for (int32_t e = 0; e < nnz_nat; e++) B.osn_pos[fill[osn[e]]++] = e;
+    }
     B.olev_ptr.assign(nlev + 1, 0);
     for (int32_t e = 0; e < nnz_nat; e++) B.olev_ptr[olev[e] + 1]++;
     for (int32_t l = 0; l < nlev; l++) B.olev_ptr[l + 1] += B.olev_ptr[l];

@@ -48,6 +48,7 @@ struct BigPlan {
     // their front
     std::vector<int64_t> nat_dest;
     std::vector<int32_t> olev_ptr, olev_pos;
+    std::vector<int32_t> osn_ptr, osn_pos;   // the same positions grouped by front
     int64_t pool = 0;                   // doubles
     int64_t inv_total = 0;
     double flops = 0.0;                 // partial-LU flops of one factorization
