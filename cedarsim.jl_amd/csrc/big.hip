@@ -826,7 +826,11 @@ struct BigCircuit {
     double factor_ms = 0.0, solve_ms = 0.0;
     bool mfma = false;   // trailing updates on the matrix cores
     bool use_graph = true;
-    bool fused = false;  // one workgroup per front and level (else multi-launch panels)
+    // per level: fused (one workgroup per front) or multi-launch panels;
+    // chosen by tune_levels unless CEDAR_MF_FRONT=fused|panel fixes it
+    std::vector<char> lvl_fused;
+    bool tuned = false;
+    std::vector<float> tune_ms_panel, tune_ms_fused;
 };
 
 namespace {
@@ -1003,39 +1007,77 @@ int upload(BigCircuit& C, std::string& err) {
     return CS_OK;
 }
 
-void launch_factor(BigCircuit& C, hipStream_t s) {
+// One assembly-tree level of the factorization, in either form:
+//   fused  one workgroup per front (gather, Jacobian entries, partial LU);
+//   panel  gather tiles, Jacobian entries, LDS fronts, then per panel of the
+//          large fronts: diagonal block, triangular solves, trailing update
+//          spread over many workgroups.
+// Both compute the same values in the same order (bit-identical fronts).
+void launch_level(BigCircuit& C, hipStream_t s, int l, bool fused) {
     const BigDev* P = C.d_dev;
-    if (C.fused) {
-        for (auto& Lv : C.levels) {
-            if (Lv.small_n)
-                hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
-                                   C.Aval, C.d_fronts + Lv.small_b);
-            if (Lv.big_n)
-                hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
-                                   C.d_fronts + Lv.big_b);
-        }
+    auto& Lv = C.levels[l];
+    if (fused) {
+        if (Lv.small_n)
+            hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv, C.Aval,
+                               C.d_fronts + Lv.small_b);
+        if (Lv.big_n)
+            hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
+                               C.d_fronts + Lv.big_b);
         return;
     }
-    for (auto& Lv : C.levels) {
-        if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
-        if (Lv.orig_e > Lv.orig_b)
-            hipLaunchKernelGGL(k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval,
-                               C.pool, Lv.orig_b, Lv.orig_e);
-        if (Lv.small_n)
-            hipLaunchKernelGGL(k_mf_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
-                               C.d_fronts + Lv.small_b);
-        for (auto& pn : Lv.panels) {
-            hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
-            if (pn.trsm_n)
-                hipLaunchKernelGGL(k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_trsm + pn.trsm_b);
-            if (pn.gemm_n) {
-                if (C.mfma)
-                    hipLaunchKernelGGL(k_mf_gemm_mfma, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
-                else
-                    hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
-            }
+    if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
+    if (Lv.orig_e > Lv.orig_b)
+        hipLaunchKernelGGL(k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval, C.pool,
+                           Lv.orig_b, Lv.orig_e);
+    if (Lv.small_n)
+        hipLaunchKernelGGL(k_mf_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
+                           C.d_fronts + Lv.small_b);
+    for (auto& pn : Lv.panels) {
+        hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
+        if (pn.trsm_n)
+            hipLaunchKernelGGL(k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_trsm + pn.trsm_b);
+        if (pn.gemm_n) {
+            if (C.mfma)
+                hipLaunchKernelGGL(k_mf_gemm_mfma, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+            else
+                hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
         }
     }
+}
+
+void launch_factor(BigCircuit& C, hipStream_t s) {
+    for (int l = 0; l < (int)C.levels.size(); l++) launch_level(C, s, l, C.lvl_fused[l] != 0);
+}
+
+// The autotuner: for every level of this pattern, time both forms (the
+// second of two runs each, on the current Jacobian values) and keep the
+// faster.  The choice changes timing only, never the factors.
+int tune_levels(BigCircuit& C, hipStream_t s, std::string& err) {
+    const int nl = (int)C.levels.size();
+    hipEvent_t ev[5];
+    for (auto& e : ev) BCHK(hipEventCreate(&e));
+    for (int l = 0; l < nl; l++) {
+        BCHK(hipEventRecord(ev[0], s));
+        launch_level(C, s, l, false);
+        BCHK(hipEventRecord(ev[1], s));
+        launch_level(C, s, l, true);
+        BCHK(hipEventRecord(ev[2], s));
+        launch_level(C, s, l, false);
+        BCHK(hipEventRecord(ev[3], s));
+        launch_level(C, s, l, true);
+        BCHK(hipEventRecord(ev[4], s));
+        BCHK(hipEventSynchronize(ev[4]));
+        float tp = 0.f, tf = 0.f;
+        BCHK(hipEventElapsedTime(&tp, ev[2], ev[3]));
+        BCHK(hipEventElapsedTime(&tf, ev[3], ev[4]));
+        C.lvl_fused[l] = tf < tp ? 1 : 0;
+        C.tune_ms_panel.push_back(tp);
+        C.tune_ms_fused.push_back(tf);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    BCHK(hipGetLastError());
+    C.tuned = true;
+    return CS_OK;
 }
 
 void launch_solve(BigCircuit& C, hipStream_t s) {
@@ -1113,6 +1155,22 @@ int set_step(BigCircuit& C, hipStream_t s, const StepDev& sp, std::string& err) 
     return CS_OK;
 }
 
+// Tune the factorization once per circuit, on a Jacobian assembled at the
+// current state (the values only need to be finite).
+int ensure_tuned(BigCircuit& C, hipStream_t s, double t, std::string& err) {
+    if (C.tuned) return CS_OK;
+    StepDev sp{};
+    sp.t = t;
+    sp.mode = CS_MODE_DCOP;
+    sp.kind = STEP_DC;
+    int rc = set_step(C, s, sp, err);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eval, dim3(grid_for(C.B.base.n_mos + C.B.base.n_src)), dim3(kBT), 0, s, C.d_dev, C.p, C.u,
+                       C.sp, C.st);
+    launch_assemble(C, s, nullptr, nullptr, nullptr, nullptr);
+    return tune_levels(C, s, err);
+}
+
 int init_point(BigCircuit& C, hipStream_t s, const double* u0, const cs_tran_opts& o, bool uic,
                cs_tran_stats* st, std::string& err) {
     const int32_t n = C.B.n;
@@ -1148,11 +1206,16 @@ int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::strin
     const char* g = std::getenv("CEDAR_MF_GEMM");
     C->mfma = !(g && std::strcmp(g, "valu") == 0);
     const char* fm = std::getenv("CEDAR_MF_FRONT");
-    C->fused = fm && std::strcmp(fm, "fused") == 0;
+    const bool fix_fused = fm && std::strcmp(fm, "fused") == 0;
+    const bool fix_panel = fm && std::strcmp(fm, "panel") == 0;
     const char* gg = std::getenv("CEDAR_MF_GRAPH");
     C->use_graph = !(gg && std::strcmp(gg, "0") == 0);
     int rc = build_bigplan(nl, C->B, err);
     if (rc == CS_OK) rc = upload(*C, err);
+    if (rc == CS_OK) {
+        C->lvl_fused.assign(C->levels.size(), fix_fused ? 1 : 0);
+        C->tuned = fix_fused || fix_panel;
+    }
     if (rc != CS_OK) {
         big_destroy(C);
         return rc;
@@ -1204,7 +1267,9 @@ void big_last_times(const BigCircuit* C, double* f, double* s) {
 int big_dcop(BigCircuit* C, hipStream_t s, const double* p, const cs_tran_opts* o, double* u_out,
              cs_tran_stats* st, std::string& err) {
     int rc;
-    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_graph(*C, s, err))) return rc;
+    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_tuned(*C, s, o->t_start, err)) ||
+        (rc = ensure_graph(*C, s, err)))
+        return rc;
     cs_tran_stats stt{};
     rc = init_point(*C, s, nullptr, *o, false, &stt, err);
     if (u_out) BCHK(hipMemcpy(u_out, C->u, sizeof(double) * C->B.n, hipMemcpyDeviceToHost));
@@ -1227,7 +1292,9 @@ int big_tran(BigCircuit* C, hipStream_t s, const double* p, const double* u0, co
     if (uic && !P.uic_ok) { err = "UIC: a charge-carrying row could not be kept on its own pivot"; return CS_E_INVALID; }
     if (o->n_out > 0 && !(o->dt_out > 0)) { err = "dt_out must be > 0"; return CS_E_INVALID; }
     int rc;
-    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_graph(*C, s, err))) return rc;
+    if ((rc = set_params(*C, s, p, err)) || (rc = ensure_tuned(*C, s, o->t_start, err)) ||
+        (rc = ensure_graph(*C, s, err)))
+        return rc;
     cs_tran_stats st{};
     rc = init_point(*C, s, u0, *o, uic, &st, err);
     const int gr = grid_for(n);
@@ -1413,6 +1480,10 @@ int big_jacobian(BigCircuit* C, hipStream_t s, double* vals, const double* du, c
 
 int big_lu_refactor(BigCircuit* C, hipStream_t s, const double* vals, std::string& err) {
     BCHK(hipMemcpyAsync(C->Aval, vals, sizeof(double) * C->B.base.nnz_nat, hipMemcpyHostToDevice, s));
+    if (!C->tuned) {
+        int rc = tune_levels(*C, s, err);
+        if (rc) return rc;
+    }
     BCHK(hipEventRecord(C->ev[0], s));
     launch_factor(*C, s);
     BCHK(hipEventRecord(C->ev[1], s));
