@@ -13,10 +13,13 @@
 //                       once, children in a fixed order: no atomics)
 //               orig    + the level's Jacobian entries (distinct positions)
 //               small   fronts with m <= kSmallM: one workgroup each, the
-//                       whole front in LDS, right-looking partial LU
-//               large   blocked right-looking partial LU, panel width kNb:
-//                       diagonal block (one workgroup), row / column
-//                       triangular solves, trailing update by 64 x 64 tiles
+//                       gather straight into LDS, right-looking partial LU
+//               large   gather and entries, then blocked right-looking
+//                       partial LU, panel width kNb: diagonal block (one
+//                       workgroup), row / column triangular solves, trailing
+//                       update by 64 x 64 MFMA tiles over many workgroups
+//                       (or one workgroup per front, where the autotuner
+//                       measured that faster for the level)
 //   solve     forward by levels (children's update vectors gathered in a
 //             fixed order), backward from the root
 //   update    u += dx, WRMS / max norms (fixed-order block reductions)
@@ -51,7 +54,9 @@ struct BigDev {
     const int32_t *sn_first, *sn_m, *sn_rptr, *sn_rows, *ch_ptr, *ch_sn, *rel_ptr, *rel, *inv, *olev_pos;
     const int32_t *osn_ptr, *osn_pos;
     const int32_t *row_long, *long_rows;   // rows too long for one lane (supply node, ...)
-    int32_t n_long, pad_;
+    const int32_t* long_cptr;              // [n_long+1] chunk range of each long row
+    const int4* long_chunks;               // (row, kind, begin, end)
+    int32_t n_long, n_chunks;
     const int64_t *sn_off, *inv_ptr, *nat_dest;
 };
 
@@ -166,42 +171,63 @@ __global__ __launch_bounds__(kBT) void k_assemble(const BigDev* P, const double*
     if (threadIdx.x == 0 && part) part[blockIdx.x] = m;
 }
 
-// A row with thousands of entries (a supply node touching every PMOS): one
-// workgroup per row, fixed-order block sums.
-__global__ __launch_bounds__(kBT) void k_assemble_long(const BigDev* P, const double* Glin, const double* Clin,
-                                                       const StepDev* spp, const double* u, const double* qx,
-                                                       const double* st, const double* pin, double* Aval,
-                                                       double* x, double* part) {
+// A row with thousands of entries (a supply node touching every PMOS) is cut
+// into chunks of up to 256 entries (or 256 residual references), one
+// workgroup per chunk; chunk = (row, kind: 0 entries / 1 residual refs,
+// begin, end).  k_long_finish adds the chunk partials of each row in a fixed
+// order.
+__global__ __launch_bounds__(kBT) void k_assemble_chunks(const BigDev* P, const double* Glin, const double* Clin,
+                                                         const StepDev* spp, const double* u, const double* st,
+                                                         const double* pin, double* Aval, double* lpart) {
     __shared__ double red[kBT];
     const StepDev sp = *spp;
     const double a0 = sp.kind == STEP_TRAN ? sp.a0 : 0.0;
+    const bool pinned_row = sp.kind == STEP_UIC && pin;
+    const int4 ch = P->long_chunks[blockIdx.x];
+    const int r = ch.x;
+    double acc = 0.0;
+    const int q = ch.z + threadIdx.x;
+    if (ch.y == 0) {
+        if (q < ch.w) {
+            if (pinned_row && P->is_diff[r]) {
+                Aval[q] = P->col[q] == r ? 1.0 : 0.0;
+            } else {
+                const double jl = Glin[q] + a0 * Clin[q];
+                acc = jl * u[P->col[q]];
+                double v = jl;
+                for (int t = P->jnl_ptr[q]; t < P->jnl_ptr[q + 1]; t++) v += sref_v(st, P->jnl_ref[t]);
+                Aval[q] = v;
+            }
+        }
+    } else if (q < ch.w) {
+        acc = sref_v(st, P->rnl_ref[q]);
+    }
+    const double t = blk_reduce(acc, red, Sum());
+    if (threadIdx.x == 0) lpart[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(64) void k_long_finish(const BigDev* P, const StepDev* spp, const double* u,
+                                                    const double* qx, const double* pin, const double* lpart,
+                                                    double* x, double* part) {
+    const StepDev sp = *spp;
     const bool use_q = sp.kind == STEP_TRAN && qx;
     const bool use_pin = sp.kind == STEP_UIC && pin;
-    const int r = P->long_rows[blockIdx.x];
-    const int e0 = P->rowptr[r], e1 = P->rowptr[r + 1];
+    const int l = blockIdx.x * 64 + threadIdx.x;
+    if (l >= P->n_long) return;
+    const int r = P->long_rows[l];
     double res;
     if (use_pin && P->is_diff[r]) {
-        for (int e = e0 + threadIdx.x; e < e1; e += kBT) Aval[e] = P->col[e] == r ? 1.0 : 0.0;
         res = u[r] - pin[r];
     } else {
-        double acc = 0.0;
-        for (int e = e0 + threadIdx.x; e < e1; e += kBT) {
-            const double jl = Glin[e] + a0 * Clin[e];
-            acc += jl * u[P->col[e]];
-            double v = jl;
-            for (int q = P->jnl_ptr[e]; q < P->jnl_ptr[e + 1]; q++) v += sref_v(st, P->jnl_ref[q]);
-            Aval[e] = v;
+        double sa = 0.0, sn = 0.0;
+        for (int c = P->long_cptr[l]; c < P->long_cptr[l + 1]; c++) {
+            if (P->long_chunks[c].y == 0) sa += lpart[c];
+            else sn += lpart[c];
         }
-        double nl = 0.0;
-        for (int q = P->rnl_ptr[r] + threadIdx.x; q < P->rnl_ptr[r + 1]; q += kBT) nl += sref_v(st, P->rnl_ref[q]);
-        const double sa = blk_reduce(acc, red, Sum());
-        const double sn = blk_reduce(nl, red, Sum());
         res = (use_q ? sa + qx[r] : sa) + sn;
     }
-    if (threadIdx.x == 0) {
-        if (x) x[P->prow[r]] = -res;
-        if (part) part[blockIdx.x] = fabs(res);
-    }
+    if (x) x[P->prow[r]] = -res;
+    if (part) part[l] = fabs(res);
 }
 
 __global__ __launch_bounds__(kBT) void k_res_reduce(const double* part, int nb, double* norm) {
@@ -290,37 +316,12 @@ __global__ __launch_bounds__(kBT) void k_mf_gather(const BigDev* P, double* pool
     }
 }
 
-__global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* Aval, double* pool, int b,
-                                                 int e) {
+__global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* Aval, double* pool,
+                                                 const int32_t* pos_list, int b, int e) {
     for (int q = b + blockIdx.x * kBT + threadIdx.x; q < e; q += gridDim.x * kBT) {
-        const int pos = P->olev_pos[q];
+        const int pos = pos_list[q];
         pool[P->nat_dest[pos]] += Aval[pos];
     }
-}
-
-// Whole front in LDS; right-looking partial LU over its k pivots.
-__global__ __launch_bounds__(kBT) void k_mf_small(const BigDev* P, double* pool, double* dinv,
-                                                  const int32_t* fronts) {
-    extern __shared__ double lds_f[];
-    double* F = lds_f;
-    const int s = fronts[blockIdx.x];
-    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
-    double* G = pool + P->sn_off[s];
-    for (int q = threadIdx.x; q < m * m; q += kBT) F[q] = G[q];
-    __syncthreads();
-    for (int p = 0; p < k; p++) {
-        const double ip = 1.0 / F[p * m + p];
-        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) F[p * m + i] *= ip;
-        if (threadIdx.x == 0) dinv[f0 + p] = ip;
-        __syncthreads();
-        const int w = m - p - 1;
-        for (int q = threadIdx.x; q < w * w; q += kBT) {
-            const int j = p + 1 + q / w, i = p + 1 + q % w;
-            F[j * m + i] -= F[p * m + i] * F[j * m + p];
-        }
-        __syncthreads();
-    }
-    for (int q = threadIdx.x; q < m * m; q += kBT) G[q] = F[q];
 }
 
 // item = (front, p0): LU of the diagonal block [p0, p0+nb) x [p0, p0+nb)
@@ -355,10 +356,14 @@ __global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, 
 }
 
 // item = (front, p0, kind, first): kind 0 solves rows [first, first+256) of
-// L21 (x U11 = a), kind 1 columns of U12 (L11 y = a, unit diagonal)
+// L21 (x U11 = a), kind 1 columns [first, first+128) of U12 (L11 y = a, unit
+// diagonal); the columns go through LDS so both loads and stores are
+// contiguous runs of a column
+constexpr int kTrsmCols = 128;
 __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, const double* dinv,
                                                  const int4* items) {
     __shared__ double D[kNb * kNb];
+    __shared__ double T[kNb][kTrsmCols + 1];
     const int4 it = items[blockIdx.x];
     const int s = it.x, p0 = it.y;
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
@@ -368,11 +373,11 @@ __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, 
         const int j = q / nb, i = q % nb;
         D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
     }
-    __syncthreads();
-    const int r = it.w + threadIdx.x;
-    if (r >= m) return;
-    double x[kNb];
     if (it.z == 0) {
+        __syncthreads();
+        const int r = it.w + threadIdx.x;
+        if (r >= m) return;
+        double x[kNb];
         for (int c = 0; c < nb; c++) x[c] = G[(int64_t)(p0 + c) * m + r];
         for (int c = 0; c < nb; c++) {
             double v = x[c];
@@ -380,15 +385,30 @@ __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, 
             x[c] = v * dinv[f0 + p0 + c];
         }
         for (int c = 0; c < nb; c++) G[(int64_t)(p0 + c) * m + r] = x[c];
-    } else {
-        double* col = G + (int64_t)r * m + p0;
-        for (int c = 0; c < nb; c++) x[c] = col[c];
+        return;
+    }
+    const int j0 = it.w;
+    const int ncol = (m - j0) < kTrsmCols ? (m - j0) : kTrsmCols;
+    for (int q = threadIdx.x; q < nb * ncol; q += kBT) {
+        const int jj = q / nb, c = q % nb;
+        T[c][jj] = G[(int64_t)(j0 + jj) * m + p0 + c];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncol) {
+        const int jj = threadIdx.x;
+        double x[kNb];
+        for (int c = 0; c < nb; c++) x[c] = T[c][jj];
         for (int c = 0; c < nb; c++) {
             double v = x[c];
             for (int c2 = 0; c2 < c; c2++) v -= D[c2 * kNb + c] * x[c2];
             x[c] = v;
         }
-        for (int c = 0; c < nb; c++) col[c] = x[c];
+        for (int c = 0; c < nb; c++) T[c][jj] = x[c];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb * ncol; q += kBT) {
+        const int jj = q / nb, c = q % nb;
+        G[(int64_t)(j0 + jj) * m + p0 + c] = T[c][jj];
     }
 }
 
@@ -722,14 +742,24 @@ __global__ __launch_bounds__(kBT) void k_predict(int n, int np, int k, double w0
     }
 }
 
-__global__ __launch_bounds__(kBT) void k_charges_long(const BigDev* P, const double* Clin, const double* v,
-                                                      double* q) {
+__global__ __launch_bounds__(kBT) void k_charges_chunks(const BigDev* P, const double* Clin, const double* v,
+                                                        double* lpart) {
     __shared__ double red[kBT];
-    const int r = P->long_rows[blockIdx.x];
+    const int4 ch = P->long_chunks[blockIdx.x];
     double acc = 0.0;
-    for (int e = P->rowptr[r] + threadIdx.x; e < P->rowptr[r + 1]; e += kBT) acc += Clin[e] * v[P->col[e]];
+    const int q = ch.z + threadIdx.x;
+    if (ch.y == 0 && q < ch.w) acc = Clin[q] * v[P->col[q]];
     const double t = blk_reduce(acc, red, Sum());
-    if (threadIdx.x == 0) q[r] = t;
+    if (threadIdx.x == 0) lpart[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(64) void k_charges_finish(const BigDev* P, const double* lpart, double* q) {
+    const int l = blockIdx.x * 64 + threadIdx.x;
+    if (l >= P->n_long) return;
+    double acc = 0.0;
+    for (int c = P->long_cptr[l]; c < P->long_cptr[l + 1]; c++)
+        if (P->long_chunks[c].y == 0) acc += lpart[c];
+    q[P->long_rows[l]] = acc;
 }
 
 __global__ __launch_bounds__(kBT) void k_charges(const BigDev* P, const double* Clin, const double* v,
@@ -797,7 +827,7 @@ struct BigCircuit {
     void* wbuf = nullptr;   // workspace
     double *p = nullptr, *linbuf = nullptr, *Glin = nullptr, *Clin = nullptr, *Aval = nullptr, *pool = nullptr;
     double *dinv = nullptr, *u = nullptr, *x = nullptr, *qx = nullptr, *u0 = nullptr, *st = nullptr;
-    double *uvec = nullptr, *part = nullptr, *norm = nullptr, *up = nullptr, *qn = nullptr;
+    double *uvec = nullptr, *part = nullptr, *norm = nullptr, *up = nullptr, *qn = nullptr, *lpart = nullptr;
     double* H[3] = {nullptr, nullptr, nullptr};
     double* Q[2] = {nullptr, nullptr};
     double* scal = nullptr;
@@ -819,6 +849,7 @@ struct BigCircuit {
     int2* d_diag = nullptr;
     int4* d_trsm = nullptr;
     int4* d_gemm = nullptr;
+    int32_t* d_borig = nullptr;   // Jacobian positions of the large fronts, by level
     void* tbuf = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -863,6 +894,15 @@ int upload(BigCircuit& C, std::string& err) {
         for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e++) w += P.jnl_ptr[e + 1] - P.jnl_ptr[e];
         if (w > 256) { row_long[r] = 1; long_rows.push_back(r); }
     }
+    std::vector<int32_t> long_cptr(1, 0);
+    std::vector<int4> long_chunks;
+    for (int32_t r : long_rows) {
+        for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e += kBT)
+            long_chunks.push_back(make_int4(r, 0, e, std::min(e + kBT, P.nat_rowptr[r + 1])));
+        for (int32_t q = P.rnl_ptr[r]; q < P.rnl_ptr[r + 1]; q += kBT)
+            long_chunks.push_back(make_int4(r, 1, q, std::min(q + kBT, P.rnl_ptr[r + 1])));
+        long_cptr.push_back((int32_t)long_chunks.size());
+    }
     for (int32_t q = 0; q < P.n_mos; q++)
         mos4.push_back(make_int4(P.mos[4 * q], P.mos[4 * q + 1], P.mos[4 * q + 2], P.mos[4 * q + 3]));
     for (auto& s : P.src) src4.push_back(make_int4(s.kind, s.off, s.len, s.dc));
@@ -877,7 +917,8 @@ int upload(BigCircuit& C, std::string& err) {
            o_rel = pack_add(by, B.rel), o_inv = pack_add(by, B.inv), o_olp = pack_add(by, B.olev_pos),
            o_soff = pack_add(by, B.sn_off), o_invp = pack_add(by, B.inv_ptr), o_nd = pack_add(by, B.nat_dest),
            o_probe = pack_add(by, P.probe), o_osp = pack_add(by, B.osn_ptr), o_osq = pack_add(by, B.osn_pos),
-           o_rl = pack_add(by, row_long), o_lr = pack_add(by, long_rows);
+           o_rl = pack_add(by, row_long), o_lr = pack_add(by, long_rows), o_lcp = pack_add(by, long_cptr),
+           o_lch = pack_add(by, long_chunks);
     BCHK(hipMalloc(&C.dbuf, by.size()));
     BCHK(hipMemcpy(C.dbuf, by.data(), by.size(), hipMemcpyHostToDevice));
     char* b = (char*)C.dbuf;
@@ -898,13 +939,14 @@ int upload(BigCircuit& C, std::string& err) {
     d.olev_pos = I(o_olp); d.sn_off = L(o_soff); d.inv_ptr = L(o_invp); d.nat_dest = L(o_nd);
     d.osn_ptr = I(o_osp); d.osn_pos = I(o_osq);
     d.row_long = I(o_rl); d.long_rows = I(o_lr); d.n_long = (int32_t)long_rows.size();
+    d.long_cptr = I(o_lcp); d.long_chunks = (const int4*)(b + o_lch); d.n_chunks = (int32_t)long_chunks.size();
     C.d_probe = (int32_t*)(b + o_probe);
     BCHK(hipMalloc(&C.d_dev, sizeof(BigDev)));
     BCHK(hipMemcpy(C.d_dev, &C.hd, sizeof(BigDev), hipMemcpyHostToDevice));
 
     // launch tables
     std::vector<int4> tiles, trsm, gemm;
-    std::vector<int32_t> fronts;
+    std::vector<int32_t> fronts, borig;
     std::vector<int2> diag;
     const int nlev = (int)B.lev_ptr.size() - 1;
     C.levels.resize(nlev);
@@ -913,14 +955,21 @@ int upload(BigCircuit& C, std::string& err) {
         Lv.tile_b = (int)tiles.size();
         int small_max = 0, m_max = 0;
         std::vector<int32_t> small, large;
+        Lv.orig_b = (int)borig.size();
         for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) {
             const int s = B.lev_sn[q], m = B.sn_m[s];
-            const int step = std::max(1, kGatherChunk / std::max(m, 1));
-            for (int j = 0; j < m; j += step) tiles.push_back(make_int4(s, j, std::min(m, j + step), 0));
-            if (m <= kSmallM) { small.push_back(s); small_max = std::max(small_max, m); }
-            else large.push_back(s);
+            if (m <= kSmallM) {
+                small.push_back(s);
+                small_max = std::max(small_max, m);
+            } else {
+                large.push_back(s);
+                const int step = std::max(1, kGatherChunk / std::max(m, 1));
+                for (int j = 0; j < m; j += step) tiles.push_back(make_int4(s, j, std::min(m, j + step), 0));
+                borig.insert(borig.end(), B.osn_pos.begin() + B.osn_ptr[s], B.osn_pos.begin() + B.osn_ptr[s + 1]);
+            }
             m_max = std::max(m_max, m);
         }
+        Lv.orig_e = (int)borig.size();
         Lv.tile_n = (int)tiles.size() - Lv.tile_b;
         Lv.small_b = (int)fronts.size();
         fronts.insert(fronts.end(), small.begin(), small.end());
@@ -933,8 +982,6 @@ int upload(BigCircuit& C, std::string& err) {
         fronts.insert(fronts.end(), large.begin(), large.end());
         Lv.big_n = (int)large.size();
         Lv.solve_lds = sizeof(double) * (size_t)std::max(m_max, 1);
-        Lv.orig_b = B.olev_ptr[l];
-        Lv.orig_e = B.olev_ptr[l + 1];
         int kmax = 0;
         for (int s : large) kmax = std::max(kmax, B.sn_first[s + 1] - B.sn_first[s]);
         for (int p0 = 0; p0 < kmax; p0 += kNb) {
@@ -947,10 +994,8 @@ int upload(BigCircuit& C, std::string& err) {
                 if (p0 >= k) continue;
                 const int p1 = std::min(p0 + kNb, k);
                 diag.push_back(make_int2(s, p0));
-                for (int r0 = p1; r0 < m; r0 += kBT) {
-                    trsm.push_back(make_int4(s, p0, 0, r0));
-                    trsm.push_back(make_int4(s, p0, 1, r0));
-                }
+                for (int r0 = p1; r0 < m; r0 += kBT) trsm.push_back(make_int4(s, p0, 0, r0));
+                for (int r0 = p1; r0 < m; r0 += kTrsmCols) trsm.push_back(make_int4(s, p0, 1, r0));
                 for (int j0 = p1; j0 < m; j0 += kTile)
                     for (int i0 = p1; i0 < m; i0 += kTile) gemm.push_back(make_int4(s, p0, i0, j0));
             }
@@ -962,7 +1007,7 @@ int upload(BigCircuit& C, std::string& err) {
     }
     std::vector<char> tb;
     size_t o_t = pack_add(tb, tiles), o_f = pack_add(tb, fronts), o_d = pack_add(tb, diag),
-           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm);
+           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm), o_bo = pack_add(tb, borig);
     BCHK(hipMalloc(&C.tbuf, tb.size()));
     BCHK(hipMemcpy(C.tbuf, tb.data(), tb.size(), hipMemcpyHostToDevice));
     char* t = (char*)C.tbuf;
@@ -971,11 +1016,13 @@ int upload(BigCircuit& C, std::string& err) {
     C.d_diag = (int2*)(t + o_d);
     C.d_trsm = (int4*)(t + o_tr);
     C.d_gemm = (int4*)(t + o_g);
+    C.d_borig = (int32_t*)(t + o_bo);
 
     // workspace
     const size_t n = B.n, nnz = P.nnz_nat;
     const size_t nd = P.lay.n_params + P.n_lin + 3 * nnz + (size_t)B.pool + 16 * n + P.n_stamp +
-                      B.rel_ptr[B.ns] + 2 * kRowBlocks + (size_t)C.hd.n_long + 16 + NRM_SLOTS + 8 +
+                      B.rel_ptr[B.ns] + 2 * kRowBlocks + (size_t)C.hd.n_long + (size_t)C.hd.n_chunks + 32 +
+                      NRM_SLOTS + 8 +
                       sizeof(StepDev) / 8 + 64;
     BCHK(hipMalloc(&C.wbuf, nd * sizeof(double)));
     BCHK(hipMemset(C.wbuf, 0, nd * sizeof(double)));
@@ -999,6 +1046,7 @@ int upload(BigCircuit& C, std::string& err) {
     C.st = take(std::max<int32_t>(P.n_stamp, 1));
     C.uvec = take(std::max<int32_t>(B.rel_ptr[B.ns], 1));
     C.part = take(2 * kRowBlocks + (size_t)C.hd.n_long + 16);
+    C.lpart = take((size_t)C.hd.n_chunks + 16);
     C.norm = take(NRM_SLOTS);
     C.scal = take(4);
     C.sp = (StepDev*)take(sizeof(StepDev) / 8 + 1);
@@ -1016,22 +1064,19 @@ int upload(BigCircuit& C, std::string& err) {
 void launch_level(BigCircuit& C, hipStream_t s, int l, bool fused) {
     const BigDev* P = C.d_dev;
     auto& Lv = C.levels[l];
+    if (Lv.small_n)
+        hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv, C.Aval,
+                           C.d_fronts + Lv.small_b);
+    if (!Lv.big_n) return;
     if (fused) {
-        if (Lv.small_n)
-            hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv, C.Aval,
-                               C.d_fronts + Lv.small_b);
-        if (Lv.big_n)
-            hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
-                               C.d_fronts + Lv.big_b);
+        hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
+                           C.d_fronts + Lv.big_b);
         return;
     }
     if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
     if (Lv.orig_e > Lv.orig_b)
         hipLaunchKernelGGL(k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval, C.pool,
-                           Lv.orig_b, Lv.orig_e);
-    if (Lv.small_n)
-        hipLaunchKernelGGL(k_mf_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv,
-                           C.d_fronts + Lv.small_b);
+                           C.d_borig, Lv.orig_b, Lv.orig_e);
     for (auto& pn : Lv.panels) {
         hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
         if (pn.trsm_n)
@@ -1057,6 +1102,12 @@ int tune_levels(BigCircuit& C, hipStream_t s, std::string& err) {
     hipEvent_t ev[5];
     for (auto& e : ev) BCHK(hipEventCreate(&e));
     for (int l = 0; l < nl; l++) {
+        if (!C.levels[l].big_n) {   // small fronts only: one form
+            launch_level(C, s, l, false);
+            C.tune_ms_panel.push_back(0.f);
+            C.tune_ms_fused.push_back(0.f);
+            continue;
+        }
         BCHK(hipEventRecord(ev[0], s));
         launch_level(C, s, l, false);
         BCHK(hipEventRecord(ev[1], s));
@@ -1097,15 +1148,21 @@ int launch_assemble(BigCircuit& C, hipStream_t s, const double* qx, const double
     const int gr = grid_for(C.B.n);
     hipLaunchKernelGGL(k_assemble, dim3(gr), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, qx, C.st, pin,
                        C.Aval, x, part);
-    if (C.hd.n_long)
-        hipLaunchKernelGGL(k_assemble_long, dim3(C.hd.n_long), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u, qx,
-                           C.st, pin, C.Aval, x, part ? part + gr : nullptr);
+    if (C.hd.n_long) {
+        hipLaunchKernelGGL(k_assemble_chunks, dim3(C.hd.n_chunks), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u,
+                           C.st, pin, C.Aval, C.lpart);
+        hipLaunchKernelGGL(k_long_finish, dim3((C.hd.n_long + 63) / 64), dim3(64), 0, s, P, C.sp, C.u, qx, pin,
+                           C.lpart, x, part ? part + gr : nullptr);
+    }
     return gr + C.hd.n_long;
 }
 
 void launch_charges(BigCircuit& C, hipStream_t s, const double* v, double* q) {
     hipLaunchKernelGGL(k_charges, dim3(grid_for(C.B.n)), dim3(kBT), 0, s, C.d_dev, C.Clin, v, q);
-    if (C.hd.n_long) hipLaunchKernelGGL(k_charges_long, dim3(C.hd.n_long), dim3(kBT), 0, s, C.d_dev, C.Clin, v, q);
+    if (C.hd.n_long) {
+        hipLaunchKernelGGL(k_charges_chunks, dim3(C.hd.n_chunks), dim3(kBT), 0, s, C.d_dev, C.Clin, v, C.lpart);
+        hipLaunchKernelGGL(k_charges_finish, dim3((C.hd.n_long + 63) / 64), dim3(64), 0, s, C.d_dev, C.lpart, q);
+    }
 }
 
 void launch_iteration(BigCircuit& C, hipStream_t s) {
