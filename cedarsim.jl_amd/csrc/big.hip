@@ -686,10 +686,27 @@ __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* p
         for (int a = threadIdx.x; a < rc; a += kBT) v[P->rel[P->rel_ptr[c] + a]] += uvec[P->rel_ptr[c] + a];
         __syncthreads();
     }
-    for (int p = 0; p < k; p++) {
-        const double y = v[p];
-        const double* col = G + (int64_t)p * m;
-        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) v[i] -= col[i] * y;
+    // blocks of 64 pivots: the diagonal block by the first wavefront, one row
+    // per lane and the pivot values by shuffle; then the rows below the block
+    // (one row per thread, the block's 64 columns of L)
+    const int lane = threadIdx.x & 63;
+    for (int b0 = 0; b0 < k; b0 += 64) {
+        const int b1 = (k - b0) < 64 ? k : b0 + 64;
+        if (threadIdx.x < 64) {
+            const int i = b0 + lane;
+            double vi = i < b1 ? v[i] : 0.0;
+            for (int p = b0; p < b1; p++) {
+                const double y = __shfl(vi, p - b0, 64);
+                if (i > p && i < b1) vi -= G[(int64_t)p * m + i] * y;
+            }
+            if (i < b1) v[i] = vi;
+        }
+        __syncthreads();
+        for (int i = b1 + threadIdx.x; i < m; i += kBT) {
+            double acc = v[i];
+            for (int p = b0; p < b1; p++) acc -= G[(int64_t)p * m + i] * v[p];
+            v[i] = acc;
+        }
         __syncthreads();
     }
     for (int a = threadIdx.x; a < m; a += kBT) {
@@ -716,12 +733,28 @@ __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* p
         w[p] = acc;
     }
     __syncthreads();
-    for (int p = k - 1; p >= 0; p--) {
-        const double xp = w[p] * dinv[f0 + p];
+    // blocks of 64 pivots from the bottom: the diagonal block of U by the
+    // first wavefront (row per lane, solution values by shuffle), then the
+    // rows above it
+    const int lane = threadIdx.x & 63;
+    for (int b1 = k; b1 > 0; b1 -= 64) {
+        const int b0 = b1 > 64 ? b1 - 64 : 0;
+        if (threadIdx.x < 64) {
+            const int i = b0 + lane;
+            double wi = i < b1 ? w[i] : 0.0;
+            for (int p = b1 - 1; p >= b0; p--) {
+                if (i == p) wi *= dinv[f0 + p];
+                const double xp = __shfl(wi, p - b0, 64);
+                if (i < p) wi -= G[(int64_t)p * m + i] * xp;
+            }
+            if (i < b1) w[i] = wi;
+        }
         __syncthreads();
-        if (threadIdx.x == 0) w[p] = xp;
-        const double* col = G + (int64_t)p * m;
-        for (int i = threadIdx.x; i < p; i += kBT) w[i] -= col[i] * xp;
+        for (int i = threadIdx.x; i < b0; i += kBT) {
+            double acc = w[i];
+            for (int p = b1 - 1; p >= b0; p--) acc -= G[(int64_t)p * m + i] * w[p];
+            w[i] = acc;
+        }
         __syncthreads();
     }
     for (int p = threadIdx.x; p < k; p += kBT) x[f0 + p] = w[p];
