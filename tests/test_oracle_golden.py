@@ -6,7 +6,9 @@ test/transients.jl, test/sweep.jl and test/sensitivity.jl, plus closed forms
 and brute-force checks for what those leave open (square-law regions, the
 BDF integrator's order, the dense LU).
 """
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -16,6 +18,10 @@ from cedarhip import (C, CurrentSource, Gnd, I, L, Named, R, SquareLawMOSFET, V,
                       net, pulse, pwl, spsin, vccs, vcvs)
 from cedarhip.workloads import inverter
 from oracle import oracle as O
+
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                   "reference_answers.json")))
 
 
 def dcop(fn):
@@ -40,9 +46,10 @@ def test_simple_vr():
         vcc = Named(net, "vcc")(); gnd = Named(net, "gnd")()
         Named(V(5.0), "V")(vcc, gnd); Named(R(2), "R")(vcc, gnd); Gnd()(gnd)
     flat, u = dcop(VR)
-    assert u[flat.index("vcc")] == pytest.approx(5.0, abs=1e-7)
+    g = GOLD["simple_vr"]
+    assert u[flat.index("vcc")] == pytest.approx(g["v_vcc"], abs=1e-7)
     # current through R = V/R; the source supplies it (branch current -2.5)
-    assert u[flat.index("V.I")] == pytest.approx(-2.5, abs=1e-7)
+    assert u[flat.index("V.I")] == pytest.approx(g["i_source"], abs=1e-7)
 
 
 def test_simple_ir():
@@ -51,7 +58,7 @@ def test_simple_ir():
         icc = Named(net, "icc")(); gnd = Named(net, "gnd")()
         Named(I(-5.0), "I")(icc, gnd); Named(R(2), "R")(icc, gnd); Gnd()(gnd)
     flat, u = dcop(IR)
-    assert u[flat.index("icc")] == pytest.approx(10.0, abs=1e-7)
+    assert u[flat.index("icc")] == pytest.approx(GOLD["simple_ir"]["v_icc"], abs=1e-7)
 
 
 def test_vrc_step():
@@ -64,10 +71,11 @@ def test_vrc_step():
     flat, o, w, st = tran(VRC, (0.0, 1.0), n_out=1001, u0=np.array([5.0, 0.0, 0.0]),
                           reltol=1e-8, abstol=1e-8)
     vc = w[:, flat.index("vrc")]
-    assert vc[0] == pytest.approx(0.0, abs=1e-7)
-    assert vc[-1] == pytest.approx(5.0, abs=1e-7)
+    g = GOLD["vrc"]
+    assert vc[0] == pytest.approx(g["vc_start"], abs=1e-7)
+    assert vc[-1] == pytest.approx(g["vc_end"], abs=1e-7)
     # C.I[1] = (V - Vc)/R at t = 0: the source current at t=0 is -V/R
-    assert w[0, flat.index("V.I")] == pytest.approx(-5.0 / 2000.0, rel=1e-7)
+    assert w[0, flat.index("V.I")] == pytest.approx(g["i_source_start"], rel=1e-7)
     # whole waveform against the closed form; a per-step local-error controller
     # at reltol 1e-8 accumulates a global error of a few ppm here
     t = o.t_start + o.dt_out * np.arange(o.n_out)
@@ -87,12 +95,13 @@ def test_spectre_sources():
         Named(R(4e3), "R4")(n[4], gnd)
         Gnd()(gnd)
     flat, o, w, st = tran(src, (0.0, 1.0), n_out=11, reltol=1e-8, abstol=1e-8)
-    assert np.allclose(w[:, flat.index("node_1")], 2.2e-3)
-    assert np.allclose(w[:, flat.index("node_3")], -1.5)
-    assert w[-1, flat.index("node_2")] == pytest.approx(3.5)
-    assert w[-1, flat.index("node_4")] == pytest.approx(-5.0)
+    g = GOLD["spectre_sources"]
+    assert np.allclose(w[:, flat.index("node_1")], g["node_1"])
+    assert np.allclose(w[:, flat.index("node_3")], g["node_3"])
+    assert w[-1, flat.index("node_2")] == pytest.approx(g["node_2_end"])
+    assert w[-1, flat.index("node_4")] == pytest.approx(g["node_4_end"])
     # R4.I[end] = V/R = -1.25 mA; R3.I = -1.5 mA
-    assert w[-1, flat.index("node_4")] / 4e3 == pytest.approx(-1.25e-3)
+    assert w[-1, flat.index("node_4")] / 4e3 == pytest.approx(g["i_r4_end"])
 
 
 def test_spice_controlled_sources():
@@ -108,11 +117,9 @@ def test_spice_controlled_sources():
         Named(vccs(2.0), "G9")(gnd, n[9], gnd, n[5]); Named(R(1e3), "R9")(n[9], gnd)
         Gnd()(gnd)
     flat, u = dcop(src)
-    assert u[flat.index("node_5")] == pytest.approx(2.0)
-    assert u[flat.index("node_6")] == pytest.approx(4.0)
-    assert u[flat.index("node_8")] == pytest.approx(4.0)
-    assert u[flat.index("node_7")] == pytest.approx(-4000.0)
-    assert u[flat.index("node_9")] == pytest.approx(-4000.0)
+    g = GOLD["spice_controlled"]
+    for k in ("node_5", "node_6", "node_8", "node_7", "node_9"):
+        assert u[flat.index(k)] == pytest.approx(g[k])
 
 
 # ---- test/transients.jl ----------------------------------------------------------
@@ -131,14 +138,15 @@ def test_pwl_current_through_resistor():
 def test_pwl_breakpoint_semantics():
     """test/transients.jl:66-96 and src/spectre_env.jl:15-21: a breakpoint belongs
     to the segment that starts there; slopes 0 before, 5e8 at 100 ns."""
-    ts = np.array([0.0, 100e-9, 110e-9, 200e-9, 210e-9])
-    ys = np.array([0.0, 0.0, 5.0, 5.0, 0.0])
+    g = GOLD["pwl_breakpoint_slopes"]
+    ts = np.array(g["ts"])
+    ys = np.array(g["ys"])
     f = lambda t: O.pwl_at_time(ts, ys, t)
     h = 1e-12
     assert (f(50e-9 + h) - f(50e-9)) / h == pytest.approx(0.0, abs=1e-6)
-    assert (f(100e-9 + h) - f(100e-9)) / h == pytest.approx(5e8, rel=1e-6)
+    assert (f(100e-9 + h) - f(100e-9)) / h == pytest.approx(g["slope_at_100ns"], rel=1e-6)
     assert (f(110e-9 + h) - f(110e-9)) / h == pytest.approx(0.0, abs=1e-6)
-    assert (f(200e-9 + h) - f(200e-9)) / h == pytest.approx(-5e8, rel=1e-6)
+    assert (f(200e-9 + h) - f(200e-9)) / h == pytest.approx(g["slope_at_200ns"], rel=1e-6)
     # infinitely steep step: the value at the step is the mean of both sides
     assert O.pwl_at_time(np.array([0.0, 1.0, 1.0, 2.0]), np.array([0.0, 0.0, 5.0, 5.0]), 1.0) == 2.5
     assert O.pwl_at_time(ts, ys, -1.0) == 0.0 and O.pwl_at_time(ts, ys, 1.0) == 0.0
@@ -179,9 +187,10 @@ def test_butterworth_sine():
     assert rc == 0
     t = o.t_start + o.dt_out * np.arange(o.n_out)
     ref = (np.exp(-t) - np.sin(t) - np.cos(t)) / 2 + 2 * np.sin(np.sqrt(3) * t / 2) / (np.sqrt(3) * np.sqrt(np.exp(t)))
-    assert np.max(np.abs(w[:, flat.index("vout")] - ref)) < 1e-6
+    g = GOLD["butterworth"]
+    assert np.max(np.abs(w[:, flat.index("vout")] - ref)) < g["tol"]
     half = w[len(t) // 2:, flat.index("vout")]
-    assert np.sqrt(np.mean(half ** 2)) == pytest.approx(0.5, abs=0.1)
+    assert np.sqrt(np.mean(half ** 2)) == pytest.approx(g["rms_second_half"], abs=0.1)
 
 
 # ---- test/sweep.jl, test/sensitivity.jl -------------------------------------------
