@@ -289,6 +289,17 @@ def grid_section(ch, L, args, eng):
         sms.append(sm)
     fm, sm = min(fms), min(sms)
     log(f"grid: dc {t_dc:.2f} s ({dc.retcode}), factor {fm:.3f} ms, solve {sm:.3f} ms")
+    # per-launch HIP-event timing of the same factor + solve (plain stream
+    # launches; per launch the minimum over three runs), summed by kernel
+    runs = [s.profile() for _ in range(3)]
+    prof = {}
+    for q, (kind, lev, _) in enumerate(runs[0]):
+        t = min(r[q][2] for r in runs)
+        e = prof.setdefault(kind, {"launches": 0, "ms": 0.0})
+        e["launches"] += 1
+        e["ms"] += t
+    for kind, e in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"grid: launches {kind:8s} {e['launches']:4d}  {e['ms']:8.3f} ms")
     o = ch.TranOptions(t_start=0.0, t_stop=args.grid_tstop, reltol=1e-6, abstol=1e-9)
     t0 = time.perf_counter()
     sol = s.tran(o, p)
@@ -301,6 +312,8 @@ def grid_section(ch, L, args, eng):
             "factor_ms": fm, "solve_ms": sm, "factor_flops": bi.flops,
             "factor_tflops": bi.flops / (fm * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK / 1e12,
             "factor_bytes_pool": 8 * bi.pool, "dcop_s": t_dc, "analysis_s": t_load,
+            "launch_profile_ms": {k: round(e["ms"], 4) for k, e in prof.items()},
+            "launch_counts": {k: e["launches"] for k, e in prof.items()},
             "tran_retcode": sol.retcode, "tran_steps": int(st.steps), "tran_newton_iters": int(st.newton_iters),
             "tran_wall_s": wall, "ms_per_newton_iter": wall * 1e3 / max(int(st.newton_iters), 1),
             "workload": f"{n}x{n} RC mesh + {args.grid_inv} square-law inverters (PULSE inputs, random "

@@ -150,6 +150,21 @@ class CircuitSystem:
         self.engine.check(self.engine.lib.cs_big_times(self.engine.ctx, self.cid, byref(f), byref(s_)))
         return f.value, s_.value
 
+    def profile(self):
+        """Per-launch timing of one factorization and one solve (supernodal
+        circuits): a list of (kind name, level, ms), in launch order."""
+        lib, ctx = self.engine.lib, self.engine.ctx
+        cnt = c_int32()
+        self.engine.check(lib.cs_big_profile(ctx, self.cid, None, None, None, 0, byref(cnt)))
+        n = cnt.value + 64
+        kinds = np.zeros(n, np.int32)
+        levels = np.zeros(n, np.int32)
+        ms = np.zeros(n, np.float32)
+        self.engine.check(lib.cs_big_profile(ctx, self.cid, L.iptr(kinds), L.iptr(levels),
+                                             ms.ctypes.data_as(L.POINTER(L.c_float)), n, byref(cnt)))
+        k = min(cnt.value, n)
+        return [(L.MFK_NAMES[kinds[q]], int(levels[q]), float(ms[q])) for q in range(k)]
+
     # -- structure --------------------------------------------------------
     def params(self) -> np.ndarray:
         p = np.zeros(self.info.n_params)

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, Structure, byref, c_char_p, c_double, c_int32, c_int64, c_void_p
+from ctypes import c_float, POINTER, Structure, byref, c_char_p, c_double, c_int32, c_int64, c_void_p
 
 import numpy as np
 
@@ -37,6 +37,8 @@ MODE_TRAN, MODE_DCOP = 0, 1
 INIT_DCOP, INIT_UIC = 0, 1
 BATCH_KERNEL_AUTO, BATCH_KERNEL_LDS, BATCH_KERNEL_GLOBAL = 0, 1, 2
 SOLVER_AUTO, SOLVER_LEVEL, SOLVER_SUPERNODAL = 0, 1, 2
+# launch kinds of cs_big_profile (CS_MFK_*)
+MFK_NAMES = ("small32", "small64", "small96", "gather", "orig", "diag", "trsm", "gemm", "front", "fwd", "bwd")
 SUPERNODAL_MIN_N = 8192
 
 _I = POINTER(c_int32)
@@ -113,6 +115,7 @@ SIGNATURES = {
     "cs_circuit_load_solver": (c_int32, [c_void_p, POINTER(cs_netlist), c_int32, _I]),
     "cs_big_info_get": (c_int32, [c_void_p, c_int32, POINTER(cs_big_info)]),
     "cs_big_times": (c_int32, [c_void_p, c_int32, _D, _D]),
+    "cs_big_profile": (c_int32, [c_void_p, c_int32, _I, _I, POINTER(c_float), c_int32, _I]),
     "cs_circuit_layout": (c_int32, [c_void_p, c_int32, POINTER(cs_param_layout)]),
     "cs_circuit_default_params": (c_int32, [c_void_p, c_int32, _D]),
     "cs_circuit_pattern": (c_int32, [c_void_p, c_int32, _I, _I]),

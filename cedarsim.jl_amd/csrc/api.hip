@@ -444,6 +444,16 @@ int cs_big_times(cs_ctx* ctx, int32_t cid, double* factor_ms, double* solve_ms) 
     return CS_OK;
 }
 
+int cs_big_profile(cs_ctx* ctx, int32_t cid, int32_t* kinds, int32_t* levels, float* ms, int32_t cap,
+                   int32_t* count) {
+    Circuit* C = get_circuit(ctx, cid);
+    if (!C || cap < 0) return fail(ctx, CS_E_INVALID, "bad argument");
+    if (!C->big) return fail(ctx, CS_E_INVALID, "not a supernodal circuit");
+    std::string err;
+    int rc = big_profile(C->big, ctx->stream, kinds, levels, ms, cap, count, err);
+    return rc ? fail(ctx, rc, err) : CS_OK;
+}
+
 int cs_circuit_info_get(cs_ctx* ctx, int32_t cid, cs_circuit_info* info) {
     Circuit* C = get_circuit(ctx, cid);
     if (!C || !info) return fail(ctx, CS_E_INVALID, "bad circuit id");

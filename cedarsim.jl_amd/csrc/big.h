@@ -38,6 +38,10 @@ int big_lu_solve(BigCircuit* b, hipStream_t s, double* x, std::string& err);
 // Unknowns recorded in waveforms (natural indices, validated by the caller).
 int big_set_probes(BigCircuit* b, const std::vector<int32_t>& idx, std::string& err);
 
+// Per-launch timing of one factorization and one solve (cs_big_profile).
+int big_profile(BigCircuit* b, hipStream_t s, int32_t* kinds, int32_t* levels, float* ms, int32_t cap,
+                int32_t* count, std::string& err);
+
 // Time of the last factorization and of the last forward+backward solve
 // (HIP events on the stream), milliseconds.
 void big_last_times(const BigCircuit* b, double* factor_ms, double* solve_ms);

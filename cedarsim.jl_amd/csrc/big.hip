@@ -172,10 +172,13 @@ __global__ __launch_bounds__(kBT) void k_assemble(const BigDev* P, const double*
 }
 
 // A row with thousands of entries (a supply node touching every PMOS) is cut
-// into chunks of up to 256 entries (or 256 residual references), one
-// workgroup per chunk; chunk = (row, kind: 0 entries / 1 residual refs,
-// begin, end).  k_long_finish adds the chunk partials of each row in a fixed
-// order.
+// into chunks of up to 256 entries, 256 residual references or 256
+// Jacobian references of one heavy entry (the supply node's diagonal takes
+// one from every device on it), one workgroup per chunk; chunk = (row,
+// kind, begin, end) with kind 0 entries, 1 residual refs, 2 + e the refs of
+// heavy entry e.  k_long_finish adds the chunk partials of each row and of
+// each heavy entry in a fixed order.
+constexpr int kHeavyRefs = 64;   // an entry with more Jacobian refs is heavy
 __global__ __launch_bounds__(kBT) void k_assemble_chunks(const BigDev* P, const double* Glin, const double* Clin,
                                                          const StepDev* spp, const double* u, const double* st,
                                                          const double* pin, double* Aval, double* lpart) {
@@ -195,37 +198,64 @@ __global__ __launch_bounds__(kBT) void k_assemble_chunks(const BigDev* P, const 
                 const double jl = Glin[q] + a0 * Clin[q];
                 acc = jl * u[P->col[q]];
                 double v = jl;
-                for (int t = P->jnl_ptr[q]; t < P->jnl_ptr[q + 1]; t++) v += sref_v(st, P->jnl_ref[t]);
+                const int t0 = P->jnl_ptr[q], t1 = P->jnl_ptr[q + 1];
+                if (t1 - t0 <= kHeavyRefs)
+                    for (int t = t0; t < t1; t++) v += sref_v(st, P->jnl_ref[t]);
                 Aval[q] = v;
             }
         }
     } else if (q < ch.w) {
-        acc = sref_v(st, P->rnl_ref[q]);
+        acc = sref_v(st, ch.y == 1 ? P->rnl_ref[q] : P->jnl_ref[q]);
     }
     const double t = blk_reduce(acc, red, Sum());
     if (threadIdx.x == 0) lpart[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(64) void k_long_finish(const BigDev* P, const StepDev* spp, const double* u,
-                                                    const double* qx, const double* pin, const double* lpart,
-                                                    double* x, double* part) {
+// One workgroup per long row: the chunk partials are staged in LDS and one
+// thread adds them in chunk order (the row's entry sums, its residual
+// references, then each heavy entry's references onto its Jacobian value).
+__global__ __launch_bounds__(kBT) void k_long_finish(const BigDev* P, const StepDev* spp, const double* u,
+                                                     const double* qx, const double* pin, const double* lpart,
+                                                     double* Aval, double* x, double* part) {
+    __shared__ double pl[kBT];
+    __shared__ int kl[kBT];
     const StepDev sp = *spp;
     const bool use_q = sp.kind == STEP_TRAN && qx;
     const bool use_pin = sp.kind == STEP_UIC && pin;
-    const int l = blockIdx.x * 64 + threadIdx.x;
-    if (l >= P->n_long) return;
+    const int l = blockIdx.x;
     const int r = P->long_rows[l];
-    double res;
-    if (use_pin && P->is_diff[r]) {
-        res = u[r] - pin[r];
-    } else {
-        double sa = 0.0, sn = 0.0;
-        for (int c = P->long_cptr[l]; c < P->long_cptr[l + 1]; c++) {
-            if (P->long_chunks[c].y == 0) sa += lpart[c];
-            else sn += lpart[c];
+    const bool pinned = use_pin && P->is_diff[r];
+    const int cb = P->long_cptr[l], ce = P->long_cptr[l + 1];
+    double sa = 0.0, sn = 0.0, hv = 0.0;
+    int he = -1;   // heavy entry being summed (value in hv)
+    for (int c0 = cb; c0 < ce; c0 += kBT) {
+        const int c = c0 + threadIdx.x;
+        if (c < ce) {
+            kl[threadIdx.x] = P->long_chunks[c].y;
+            pl[threadIdx.x] = lpart[c];
         }
-        res = (use_q ? sa + qx[r] : sa) + sn;
+        __syncthreads();
+        if (threadIdx.x == 0 && !pinned) {
+            const int nc = (ce - c0) < kBT ? ce - c0 : kBT;
+            for (int q = 0; q < nc; q++) {
+                const int kind = kl[q];
+                if (kind == 0) sa += pl[q];
+                else if (kind == 1) sn += pl[q];
+                else {
+                    if (kind - 2 != he) {
+                        if (he >= 0) Aval[he] = hv;
+                        he = kind - 2;
+                        hv = Aval[he];
+                    }
+                    hv += pl[q];
+                }
+            }
+        }
+        __syncthreads();
     }
+    if (threadIdx.x != 0) return;
+    if (he >= 0) Aval[he] = hv;
+    const double res = pinned ? u[r] - pin[r] : (use_q ? sa + qx[r] : sa) + sn;
     if (x) x[P->prow[r]] = -res;
     if (part) part[l] = fabs(res);
 }
@@ -293,27 +323,66 @@ __global__ __launch_bounds__(kBT) void k_upd_reduce(const BigDev* P, const StepD
 // Front s: m x m column-major at pool + sn_off[s]; k = pivots; CB = trailing r x r.
 __device__ __forceinline__ int sn_k(const BigDev* P, int s) { return P->sn_first[s + 1] - P->sn_first[s]; }
 
-// tile = (front, first column, end column): entries = sum of children's CB entries
-__global__ __launch_bounds__(kBT) void k_mf_gather(const BigDev* P, double* pool, const int4* tiles) {
+// Global-memory phases of one workgroup: stores complete and the CU's L1 is
+// invalidated before any wavefront reads what another one wrote.
+__device__ __forceinline__ void gsync() {
+    __threadfence();
+    __syncthreads();
+}
+
+// Columns [j0, j1) of front s = the sum of its children's contribution
+// blocks, children in order, into F (column j0 at F[0], leading dimension
+// m): the columns are zeroed, then each child's block is added (one
+// wavefront per block column, lanes down the rows), with a barrier between
+// children.  Update row a of child c sits at row rel[a] of the parent.  br,
+// when given, holds per child the range of its block columns that land in
+// [j0, j1) (rel is ascending); without it every column does.  Every entry
+// sees its children's values in the same order as a sum over the children.
+// NT threads; Global: F in global memory (fenced barriers), else in LDS.
+template <int NT, bool Global>
+__device__ __forceinline__ void scatter_cb(const BigDev* P, const double* pool, double* F, int m, int s, int j0,
+                                           int j1, const int2* br) {
+    for (int q = threadIdx.x; q < (j1 - j0) * m; q += NT) F[q] = 0.0;
+    if (Global) gsync();
+    else __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
+    for (int cq = c0; cq < c1; cq++) {
+        const int c = P->ch_sn[cq];
+        const int mc = P->sn_m[c], kc = sn_k(P, c), rc = mc - kc;
+        const int* rl = P->rel + P->rel_ptr[c];
+        const double* blk = pool + P->sn_off[c] + (int64_t)kc * mc + kc;
+        int b0 = 0, b1 = rc;
+        if (br) {
+            const int2 r = br[cq - c0];
+            b0 = r.x;
+            b1 = r.y;
+        }
+        for (int b = b0 + wave; b < b1; b += NT / 64) {
+            double* Fc = F + (int64_t)(rl[b] - j0) * m;
+            const double* src = blk + (int64_t)b * mc;
+            for (int a = lane; a < rc; a += 64) Fc[rl[a]] += src[a];
+        }
+        if (Global) gsync();
+        else __syncthreads();
+    }
+}
+
+// tile = (front, first column, end column, offset of its per-child column
+// ranges in cbr): the tile (at most kGatherChunk entries) is summed in LDS
+// and written once
+__global__ __launch_bounds__(kBT) void k_mf_gather(const BigDev* P, double* pool, const int4* tiles,
+                                                   const int2* cbr) {
+    __shared__ double Ft[kGatherChunk];
     const int4 t = tiles[blockIdx.x];
     const int s = t.x, m = P->sn_m[s];
-    double* F = pool + P->sn_off[s];
-    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
-    const int cnt = (t.z - t.y) * m;
-    for (int q = threadIdx.x; q < cnt; q += kBT) {
-        const int j = t.y + q / m, i = q % m;
-        double v = 0.0;
-        for (int cq = c0; cq < c1; cq++) {
-            const int c = P->ch_sn[cq];
-            const int* iv = P->inv + P->inv_ptr[c];
-            const int a = iv[i], b = iv[j];
-            if (a >= 0 && b >= 0) {
-                const int mc = P->sn_m[c], kc = sn_k(P, c);
-                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
-            }
-        }
-        F[(int64_t)j * m + i] = v;
+    double* G = pool + P->sn_off[s] + (int64_t)t.y * m;
+    if ((t.z - t.y) * m > kGatherChunk) {   // a single column longer than the LDS tile
+        scatter_cb<kBT, true>(P, pool, G, m, s, t.y, t.z, cbr + t.w);
+        return;
     }
+    scatter_cb<kBT, false>(P, pool, Ft, m, s, t.y, t.z, cbr + t.w);
+    for (int q = threadIdx.x; q < (t.z - t.y) * m; q += kBT) G[q] = Ft[q];
 }
 
 __global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* Aval, double* pool,
@@ -324,9 +393,55 @@ __global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* 
     }
 }
 
+// LU without pivoting of the nb x nb block D (column-major, leading
+// dimension kNb, in LDS) by the kBT threads of a workgroup, each holding four
+// entries in registers: thread t owns column t / 8, rows t % 8 + 8v.  Before
+// step p the owners of column p and of row p publish them in cb / rb
+// (double-buffered, one barrier per pivot).  The operations and their order
+// are those of the column sweep: l(i,p) = a(i,p) * (1 / u(p,p)), then a(i,j)
+// -= l(i,p) u(p,j) for pivots p ascending.  L (unit diagonal, below) and U
+// overwrite D; ipv[p] = 1 / u(p,p).
+__device__ __forceinline__ void diag_lu_block(double* D, double* ipv, int nb, double (*cb)[kNb],
+                                              double (*rb)[kNb]) {
+    const int t = threadIdx.x, j = t >> 3, i0 = t & 7;
+    double a[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int i = i0 + 8 * v;
+        a[v] = (i < nb && j < nb) ? D[j * kNb + i] : 0.0;
+        if (j == 0) cb[0][i] = a[v];
+        if (i == 0) rb[0][j] = a[v];
+    }
+    __syncthreads();
+    for (int p = 0; p < nb; p++) {
+        const int b = p & 1;
+        const double ip = 1.0 / rb[b][p];
+        if (t == 0) ipv[p] = ip;
+        const double u = rb[b][j];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = i0 + 8 * v;
+            if (i > p) {
+                const double l = cb[b][i] * ip;
+                if (j == p) a[v] = l;
+                else if (j > p) a[v] = a[v] - l * u;
+            }
+            if (j == p + 1) cb[b ^ 1][i] = a[v];
+            if (i == p + 1) rb[b ^ 1][j] = a[v];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int i = i0 + 8 * v;
+        if (i < nb && j < nb) D[j * kNb + i] = a[v];
+    }
+}
+
 // item = (front, p0): LU of the diagonal block [p0, p0+nb) x [p0, p0+nb)
 __global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, double* dinv, const int2* items) {
     __shared__ double D[kNb * kNb];
+    __shared__ double cb[2][kNb], rb[2][kNb];
     const int2 it = items[blockIdx.x];
     const int s = it.x, p0 = it.y;
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
@@ -337,22 +452,54 @@ __global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, 
         D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
     }
     __syncthreads();
-    for (int p = 0; p < nb; p++) {
-        const double ip = 1.0 / D[p * kNb + p];
-        for (int i = p + 1 + threadIdx.x; i < nb; i += kBT) D[p * kNb + i] *= ip;
-        if (threadIdx.x == 0) dinv[f0 + p0 + p] = ip;
-        __syncthreads();
-        const int w = nb - p - 1;
-        for (int q = threadIdx.x; q < w * w; q += kBT) {
-            const int j = p + 1 + q / w, i = p + 1 + q % w;
-            D[j * kNb + i] -= D[p * kNb + i] * D[j * kNb + p];
-        }
-        __syncthreads();
-    }
+    diag_lu_block(D, dinv + f0 + p0, nb, cb, rb);
+    __syncthreads();
     for (int q = threadIdx.x; q < nb * nb; q += kBT) {
         const int j = q / nb, i = q % nb;
         G[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
     }
+}
+
+// Triangular solve of one right-hand side held in registers, right-looking:
+// a[i] is the pending value of column c + i while column c is eliminated,
+// so every register index is fixed (a runtime-indexed array would live in
+// scratch).  The coefficient coupling column c to column c + 1 + i is
+// E[c * (kNb + 1) + 1 + i]; the kNb - 1 coefficients of a step are all
+// requested before the first is used.  Column c + 1 + i receives its
+// updates in ascending c, the order of the left-looking dot product.  Each
+// result, scaled by sc[c] when Scale, goes to put(c, x).  Columns past nb
+// carry values that are never stored.
+template <bool Scale, class Put>
+__device__ __forceinline__ void rot_solve(double (&a)[kNb], const double* E, const double* sc, int nb, Put put) {
+    for (int c = 0; c < nb; c++) {
+        const double* e = E + c * (kNb + 1) + 1;
+        double ev[kNb - 1];
+#pragma unroll
+        for (int i = 0; i < kNb - 1; i++) ev[i] = e[i];
+        const double s = Scale ? sc[c] : 1.0;
+        __builtin_amdgcn_sched_barrier(0);
+        const double xc = Scale ? a[0] * s : a[0];
+        put(c, xc);
+#pragma unroll
+        for (int i = 0; i < kNb - 1; i++) a[i] = a[i + 1] - xc * ev[i];
+        a[kNb - 1] = 0.0;
+    }
+}
+// Coefficient block of rot_solve: E[c kNb + j] (kNb + 1 rows of kNb, zero
+// outside the nb x nb block, so reads past a row stay in bounds), from the
+// factored diagonal block B (column-major, leading dimension ld):
+//   rows of L21 (x U11 = a):    E[c kNb + j] = U(c, j) = B[j ld + c]
+//   columns of U12 (L11 y = a): E[c kNb + j] = L(j, c) = B[c ld + j]
+constexpr int kCoef = kNb * (kNb + 1);
+template <bool Upper>
+__device__ __forceinline__ void load_coef(double* E, const double* B, int64_t ld, int nb) {
+    for (int q = threadIdx.x; q < kNb * kNb; q += kBT) {
+        // the index that runs fastest in B runs fastest over the threads
+        const int f = q % kNb, g = q / kNb;
+        const int c = Upper ? f : g, j = Upper ? g : f;
+        E[c * kNb + j] = (c < nb && j < nb) ? B[(int64_t)g * ld + f] : 0.0;
+    }
+    if ((int)threadIdx.x < kNb) E[kNb * kNb + threadIdx.x] = 0.0;
 }
 
 // item = (front, p0, kind, first): kind 0 solves rows [first, first+256) of
@@ -362,31 +509,28 @@ __global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, 
 constexpr int kTrsmCols = 128;
 __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, const double* dinv,
                                                  const int4* items) {
-    __shared__ double D[kNb * kNb];
+    __shared__ double E[kCoef];
+    __shared__ double sc[kNb];
     __shared__ double T[kNb][kTrsmCols + 1];
     const int4 it = items[blockIdx.x];
     const int s = it.x, p0 = it.y;
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int nb = (k - p0) < kNb ? (k - p0) : kNb;
     double* G = pool + P->sn_off[s];
-    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
-        const int j = q / nb, i = q % nb;
-        D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
-    }
+    const double* B = G + (int64_t)p0 * m + p0;
     if (it.z == 0) {
+        load_coef<true>(E, B, m, nb);
+        if ((int)threadIdx.x < nb) sc[threadIdx.x] = dinv[f0 + p0 + threadIdx.x];
         __syncthreads();
         const int r = it.w + threadIdx.x;
         if (r >= m) return;
-        double x[kNb];
-        for (int c = 0; c < nb; c++) x[c] = G[(int64_t)(p0 + c) * m + r];
-        for (int c = 0; c < nb; c++) {
-            double v = x[c];
-            for (int c2 = 0; c2 < c; c2++) v -= x[c2] * D[c * kNb + c2];
-            x[c] = v * dinv[f0 + p0 + c];
-        }
-        for (int c = 0; c < nb; c++) G[(int64_t)(p0 + c) * m + r] = x[c];
+        double a[kNb];
+#pragma unroll
+        for (int c = 0; c < kNb; c++) a[c] = c < nb ? G[(int64_t)(p0 + c) * m + r] : 0.0;
+        rot_solve<true>(a, E, sc, nb, [&](int c, double x) { G[(int64_t)(p0 + c) * m + r] = x; });
         return;
     }
+    load_coef<false>(E, B, m, nb);
     const int j0 = it.w;
     const int ncol = (m - j0) < kTrsmCols ? (m - j0) : kTrsmCols;
     for (int q = threadIdx.x; q < nb * ncol; q += kBT) {
@@ -396,14 +540,10 @@ __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, 
     __syncthreads();
     if ((int)threadIdx.x < ncol) {
         const int jj = threadIdx.x;
-        double x[kNb];
-        for (int c = 0; c < nb; c++) x[c] = T[c][jj];
-        for (int c = 0; c < nb; c++) {
-            double v = x[c];
-            for (int c2 = 0; c2 < c; c2++) v -= D[c2 * kNb + c] * x[c2];
-            x[c] = v;
-        }
-        for (int c = 0; c < nb; c++) T[c][jj] = x[c];
+        double a[kNb];
+#pragma unroll
+        for (int c = 0; c < kNb; c++) a[c] = c < nb ? T[c][jj] : 0.0;
+        rot_solve<false>(a, E, nullptr, nb, [&](int c, double x) { T[c][jj] = x; });
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nb * ncol; q += kBT) {
@@ -492,58 +632,89 @@ __global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* p
 }
 
 // ---- fused per-level front kernels (one workgroup per front) ------------------------------
-// Small fronts: children's contribution blocks and the front's Jacobian
-// entries are gathered straight into LDS, factored there and written once.
-__global__ __launch_bounds__(kBT) void k_mf_lvl_small(const BigDev* P, double* pool, double* dinv,
-                                                      const double* Aval, const int32_t* fronts) {
+// Small fronts (m <= TL * R): children's contribution blocks and the
+// front's Jacobian entries are gathered into LDS, then the partial LU runs
+// with the front in registers: thread t of the TL x TL threads holds rows
+// t % TL + TL y and columns t / TL + TL x.  Before pivot p the owners of
+// column p and row p publish them in cb / rb (double-buffered, one barrier
+// per pivot); the operations and their order are those of the column
+// sweep (l(i,p) = a(i,p) * (1 / u(p,p)), then a(i,j) -= l(i,p) u(p,j),
+// pivots ascending).  Fronts are bucketed by size so that the small ones
+// run as single wavefronts with few registers.
+template <int TL, int R>
+__global__ __launch_bounds__(TL * TL) void k_mf_lvl_small(const BigDev* P, double* pool, double* dinv,
+                                                          const double* Aval, const int32_t* fronts) {
+    constexpr int NT = TL * TL, MX = TL * R;
     extern __shared__ double lds_f[];
+    __shared__ double cb[2][MX], rb[2][MX];
     double* F = lds_f;
     const int s = fronts[blockIdx.x];
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int64_t off = P->sn_off[s];
-    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
-    for (int q = threadIdx.x; q < m * m; q += kBT) {
-        const int j = q / m, i = q % m;
-        double v = 0.0;
-        for (int cq = c0; cq < c1; cq++) {
-            const int c = P->ch_sn[cq];
-            const int* iv = P->inv + P->inv_ptr[c];
-            const int a = iv[i], b = iv[j];
-            if (a >= 0 && b >= 0) {
-                const int mc = P->sn_m[c], kc = sn_k(P, c);
-                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
-            }
-        }
-        F[q] = v;
-    }
-    __syncthreads();
-    for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += kBT) {
+    scatter_cb<NT, false>(P, pool, F, m, s, 0, m, nullptr);
+    for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += NT) {
         const int e = P->osn_pos[q];
         F[P->nat_dest[e] - off] += Aval[e];
     }
     __syncthreads();
+    const int ti = threadIdx.x % TL, tj = threadIdx.x / TL;
+    double a[R][R];
+#pragma unroll
+    for (int y = 0; y < R; y++)
+#pragma unroll
+        for (int x = 0; x < R; x++) {
+            const int i = ti + TL * y, j = tj + TL * x;
+            a[y][x] = (i < m && j < m) ? F[j * m + i] : 0.0;
+            if (j == 0) cb[0][i] = a[y][x];
+            if (i == 0) rb[0][j] = a[y][x];
+        }
+    __syncthreads();
     for (int p = 0; p < k; p++) {
-        const double ip = 1.0 / F[p * m + p];
-        for (int i = p + 1 + threadIdx.x; i < m; i += kBT) F[p * m + i] *= ip;
+        const int b = p & 1;
+        const double ip = 1.0 / rb[b][p];
         if (threadIdx.x == 0) dinv[f0 + p] = ip;
-        __syncthreads();
-        const int w = m - p - 1;
-        for (int q = threadIdx.x; q < w * w; q += kBT) {
-            const int j = p + 1 + q / w, i = p + 1 + q % w;
-            F[j * m + i] -= F[p * m + i] * F[j * m + p];
+        double l[R], u[R];
+#pragma unroll
+        for (int y = 0; y < R; y++) {
+            const int i = ti + TL * y;
+            l[y] = (i > p && i < m) ? cb[b][i] * ip : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < R; x++) {
+            const int j = tj + TL * x;
+            u[x] = (j > p && j < m) ? rb[b][j] : 0.0;
+        }
+#pragma unroll
+        for (int y = 0; y < R; y++) {
+            if (TL * y >= m) break;
+            const int i = ti + TL * y;
+#pragma unroll
+            for (int x = 0; x < R; x++) {
+                if (TL * x >= m) break;
+                const int j = tj + TL * x;
+                if (i > p) {
+                    if (j == p) a[y][x] = l[y];
+                    else if (j > p) a[y][x] = a[y][x] - l[y] * u[x];
+                }
+                if (j == p + 1) cb[b ^ 1][i] = a[y][x];
+                if (i == p + 1) rb[b ^ 1][j] = a[y][x];
+            }
         }
         __syncthreads();
     }
     double* G = pool + off;
-    for (int q = threadIdx.x; q < m * m; q += kBT) G[q] = F[q];
+#pragma unroll
+    for (int y = 0; y < R; y++)
+#pragma unroll
+        for (int x = 0; x < R; x++) {
+            const int i = ti + TL * y, j = tj + TL * x;
+            if (i < m && j < m) G[(int64_t)j * m + i] = a[y][x];
+        }
 }
-
-// Global-memory phases of one workgroup: stores complete and the CU's L1 is
-// invalidated before any wavefront reads what another one wrote.
-__device__ __forceinline__ void gsync() {
-    __threadfence();
-    __syncthreads();
-}
+// size buckets: (threads per side, entries per side per thread)
+constexpr int kSmallBuckets = 3;
+constexpr int kBucketMax[kSmallBuckets] = {32, 64, kSmallM};
+static_assert(8 * 4 >= 32 && 16 * 4 >= 64 && 16 * 6 >= kSmallM, "small-front buckets exceed their kernels");
 
 // Large fronts, one workgroup each: gather, then blocked right-looking
 // partial LU in global memory (L2-resident): diagonal block in LDS, row and
@@ -552,29 +723,16 @@ __device__ __forceinline__ void gsync() {
 __global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* pool, double* dinv,
                                                     const double* Aval, const int32_t* fronts) {
     __shared__ double D[kNb * kNb];
+    __shared__ double EU[kCoef], EL[kCoef];
     __shared__ double invd[kNb];
+    __shared__ double cb[2][kNb], rb[2][kNb];
     __shared__ double A[kNb][kTile + 1];
     __shared__ double Bt[kNb][kTile + 1];
     const int s = fronts[blockIdx.x];
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int64_t off = P->sn_off[s];
     double* G = pool + off;
-    const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
-    for (int q = threadIdx.x; q < m * m; q += kBT) {
-        const int j = q / m, i = q % m;
-        double v = 0.0;
-        for (int cq = c0; cq < c1; cq++) {
-            const int c = P->ch_sn[cq];
-            const int* iv = P->inv + P->inv_ptr[c];
-            const int a = iv[i], b = iv[j];
-            if (a >= 0 && b >= 0) {
-                const int mc = P->sn_m[c], kc = sn_k(P, c);
-                v += pool[P->sn_off[c] + (int64_t)(kc + b) * mc + (kc + a)];
-            }
-        }
-        G[q] = v;
-    }
-    gsync();
+    scatter_cb<kBT, true>(P, pool, G, m, s, 0, m, nullptr);
     for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += kBT) {
         const int e = P->osn_pos[q];
         G[P->nat_dest[e] - off] += Aval[e];
@@ -592,46 +750,29 @@ __global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* poo
             D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
         }
         __syncthreads();
-        for (int p = 0; p < nb; p++) {
-            const double ip = 1.0 / D[p * kNb + p];
-            for (int i = p + 1 + threadIdx.x; i < nb; i += kBT) D[p * kNb + i] *= ip;
-            if (threadIdx.x == 0) {
-                dinv[f0 + p0 + p] = ip;
-                invd[p] = ip;
-            }
-            __syncthreads();
-            const int w = nb - p - 1;
-            for (int q = threadIdx.x; q < w * w; q += kBT) {
-                const int j = p + 1 + q / w, i = p + 1 + q % w;
-                D[j * kNb + i] -= D[p * kNb + i] * D[j * kNb + p];
-            }
-            __syncthreads();
-        }
+        diag_lu_block(D, invd, nb, cb, rb);
+        __syncthreads();
         for (int q = threadIdx.x; q < nb * nb; q += kBT) {
             const int j = q / nb, i = q % nb;
             G[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
         }
-        // L21 rows and U12 columns (the diagonal block from LDS)
+        if ((int)threadIdx.x < nb) dinv[f0 + p0 + threadIdx.x] = invd[threadIdx.x];
+        load_coef<true>(EU, D, kNb, nb);
+        load_coef<false>(EL, D, kNb, nb);
+        __syncthreads();
+        // L21 rows and U12 columns
         for (int r = p1 + threadIdx.x; r < m; r += kBT) {
-            double x[kNb];
-            for (int c = 0; c < nb; c++) x[c] = G[(int64_t)(p0 + c) * m + r];
-            for (int c = 0; c < nb; c++) {
-                double v = x[c];
-                for (int c2 = 0; c2 < c; c2++) v -= x[c2] * D[c * kNb + c2];
-                x[c] = v * invd[c];
-            }
-            for (int c = 0; c < nb; c++) G[(int64_t)(p0 + c) * m + r] = x[c];
+            double a[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; c++) a[c] = c < nb ? G[(int64_t)(p0 + c) * m + r] : 0.0;
+            rot_solve<true>(a, EU, invd, nb, [&](int c, double x) { G[(int64_t)(p0 + c) * m + r] = x; });
         }
         for (int r = p1 + threadIdx.x; r < m; r += kBT) {
             double* col = G + (int64_t)r * m + p0;
-            double x[kNb];
-            for (int c = 0; c < nb; c++) x[c] = col[c];
-            for (int c = 0; c < nb; c++) {
-                double v = x[c];
-                for (int c2 = 0; c2 < c; c2++) v -= D[c2 * kNb + c] * x[c2];
-                x[c] = v;
-            }
-            for (int c = 0; c < nb; c++) col[c] = x[c];
+            double a[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; c++) a[c] = c < nb ? col[c] : 0.0;
+            rot_solve<false>(a, EL, nullptr, nb, [&](int c, double x) { col[c] = x; });
         }
         gsync();
         // trailing update
@@ -673,8 +814,9 @@ __global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* poo
 // ---- multifrontal solves ----------------------------------------------------------------
 // forward: y = L^-1 b by levels; update vectors of the fronts in uvec
 __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
-                                                const int32_t* fronts) {
+                                                const int32_t* fronts, int ld_off) {
     extern __shared__ double v[];
+    double* Ld = v + ld_off;   // the diagonal block, row stride 65
     const int s = fronts[blockIdx.x];
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const double* G = pool + P->sn_off[s];
@@ -686,24 +828,32 @@ __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* p
         for (int a = threadIdx.x; a < rc; a += kBT) v[P->rel[P->rel_ptr[c] + a]] += uvec[P->rel_ptr[c] + a];
         __syncthreads();
     }
-    // blocks of 64 pivots: the diagonal block by the first wavefront, one row
-    // per lane and the pivot values by shuffle; then the rows below the block
-    // (one row per thread, the block's 64 columns of L)
+    // blocks of 64 pivots: the diagonal block (staged in LDS) by the first
+    // wavefront, one row per lane and the pivot values by shuffle; then the
+    // rows below the block (one row per thread, the block's 64 columns of L,
+    // loads unrolled so that several are in flight)
     const int lane = threadIdx.x & 63;
     for (int b0 = 0; b0 < k; b0 += 64) {
         const int b1 = (k - b0) < 64 ? k : b0 + 64;
+        const int nbk = b1 - b0;
+        for (int q = threadIdx.x; q < nbk * 64; q += kBT) {
+            const int pp = q >> 6, ii = q & 63;
+            if (pp < nbk) Ld[pp * 65 + ii] = ii < nbk ? G[(int64_t)(b0 + pp) * m + b0 + ii] : 0.0;
+        }
+        __syncthreads();
         if (threadIdx.x < 64) {
             const int i = b0 + lane;
             double vi = i < b1 ? v[i] : 0.0;
             for (int p = b0; p < b1; p++) {
                 const double y = __shfl(vi, p - b0, 64);
-                if (i > p && i < b1) vi -= G[(int64_t)p * m + i] * y;
+                if (i > p && i < b1) vi -= Ld[(p - b0) * 65 + lane] * y;
             }
             if (i < b1) v[i] = vi;
         }
         __syncthreads();
         for (int i = b1 + threadIdx.x; i < m; i += kBT) {
             double acc = v[i];
+#pragma unroll 8
             for (int p = b0; p < b1; p++) acc -= G[(int64_t)p * m + i] * v[p];
             v[i] = acc;
         }
@@ -717,8 +867,9 @@ __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* p
 
 // backward: x = U^-1 y from the root down
 __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* pool, const double* dinv,
-                                                double* x, const int32_t* fronts) {
+                                                double* x, const int32_t* fronts, int ld_off) {
     extern __shared__ double w[];
+    double* Ud = w + ld_off;   // the diagonal block, row stride 65
     const int s = fronts[blockIdx.x];
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int r = m - k;
@@ -729,29 +880,37 @@ __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* p
     __syncthreads();
     for (int p = threadIdx.x; p < k; p += kBT) {
         double acc = x[f0 + p];
+#pragma unroll 8
         for (int a = 0; a < r; a++) acc -= G[(int64_t)(k + a) * m + p] * xu[a];
         w[p] = acc;
     }
     __syncthreads();
-    // blocks of 64 pivots from the bottom: the diagonal block of U by the
-    // first wavefront (row per lane, solution values by shuffle), then the
-    // rows above it
+    // blocks of 64 pivots from the bottom: the diagonal block of U (staged in
+    // LDS) by the first wavefront (row per lane, solution values by
+    // shuffle), then the rows above it
     const int lane = threadIdx.x & 63;
     for (int b1 = k; b1 > 0; b1 -= 64) {
         const int b0 = b1 > 64 ? b1 - 64 : 0;
+        const int nbk = b1 - b0;
+        for (int q = threadIdx.x; q < nbk * 64; q += kBT) {
+            const int pp = q >> 6, ii = q & 63;
+            Ud[pp * 65 + ii] = ii < nbk ? G[(int64_t)(b0 + pp) * m + b0 + ii] : 0.0;
+        }
+        __syncthreads();
         if (threadIdx.x < 64) {
             const int i = b0 + lane;
             double wi = i < b1 ? w[i] : 0.0;
             for (int p = b1 - 1; p >= b0; p--) {
                 if (i == p) wi *= dinv[f0 + p];
                 const double xp = __shfl(wi, p - b0, 64);
-                if (i < p) wi -= G[(int64_t)p * m + i] * xp;
+                if (i < p) wi -= Ud[(p - b0) * 65 + lane] * xp;
             }
             if (i < b1) w[i] = wi;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < b0; i += kBT) {
             double acc = w[i];
+#pragma unroll 8
             for (int p = b1 - 1; p >= b0; p--) acc -= G[(int64_t)p * m + i] * w[p];
             w[i] = acc;
         }
@@ -786,13 +945,27 @@ __global__ __launch_bounds__(kBT) void k_charges_chunks(const BigDev* P, const d
     if (threadIdx.x == 0) lpart[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(64) void k_charges_finish(const BigDev* P, const double* lpart, double* q) {
-    const int l = blockIdx.x * 64 + threadIdx.x;
-    if (l >= P->n_long) return;
+__global__ __launch_bounds__(kBT) void k_charges_finish(const BigDev* P, const double* lpart, double* q) {
+    __shared__ double pl[kBT];
+    __shared__ int kl[kBT];
+    const int l = blockIdx.x;
+    const int cb = P->long_cptr[l], ce = P->long_cptr[l + 1];
     double acc = 0.0;
-    for (int c = P->long_cptr[l]; c < P->long_cptr[l + 1]; c++)
-        if (P->long_chunks[c].y == 0) acc += lpart[c];
-    q[P->long_rows[l]] = acc;
+    for (int c0 = cb; c0 < ce; c0 += kBT) {
+        const int c = c0 + threadIdx.x;
+        if (c < ce) {
+            kl[threadIdx.x] = P->long_chunks[c].y;
+            pl[threadIdx.x] = lpart[c];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nc = (ce - c0) < kBT ? ce - c0 : kBT;
+            for (int t = 0; t < nc; t++)
+                if (kl[t] == 0) acc += pl[t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) q[P->long_rows[l]] = acc;
 }
 
 __global__ __launch_bounds__(kBT) void k_charges(const BigDev* P, const double* Clin, const double* v,
@@ -872,12 +1045,15 @@ struct BigCircuit {
         int diag_b, diag_n, trsm_b, trsm_n, gemm_b, gemm_n;
     };
     struct Level {
-        int tile_b, tile_n, small_b, small_n, fr_b, fr_n, orig_b, orig_e, big_b, big_n;
-        size_t small_lds, solve_lds;
+        int tile_b, tile_n, fr_b, fr_n, orig_b, orig_e, big_b, big_n;
+        int small_b[kSmallBuckets], small_n[kSmallBuckets];   // fronts up to kBucketMax[bucket]
+        size_t small_lds[kSmallBuckets], solve_lds;
+        int ld_off;   // offset of the solves' diagonal block in their LDS
         std::vector<Panel> panels;
     };
     std::vector<Level> levels;
     int4* d_tiles = nullptr;
+    int2* d_cbr = nullptr;
     int32_t* d_fronts = nullptr;
     int2* d_diag = nullptr;
     int4* d_trsm = nullptr;
@@ -895,7 +1071,32 @@ struct BigCircuit {
     std::vector<char> lvl_fused;
     bool tuned = false;
     std::vector<float> tune_ms_panel, tune_ms_fused;
+    // per-launch timing (cs_big_profile): an event pair around each launch
+    struct LaunchProf {
+        std::vector<hipEvent_t> ev;
+        std::vector<int32_t> kind, level;
+    };
+    LaunchProf* prof = nullptr;
 };
+
+// Launch of a factor or solve kernel; under cs_big_profile it is bracketed
+// by two events on the stream.
+void prof_mark(BigCircuit& C, hipStream_t s, int kind, int level) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    if (kind >= 0) {
+        C.prof->kind.push_back(kind);
+        C.prof->level.push_back(level);
+    }
+    C.prof->ev.push_back(e);
+}
+#define MF_LAUNCH(kind, lvl, ...)                                   \
+    do {                                                            \
+        if (C.prof) prof_mark(C, s, kind, lvl);                     \
+        hipLaunchKernelGGL(__VA_ARGS__);                            \
+        if (C.prof) prof_mark(C, s, -1, lvl);                       \
+    } while (0)
 
 namespace {
 
@@ -934,6 +1135,11 @@ int upload(BigCircuit& C, std::string& err) {
             long_chunks.push_back(make_int4(r, 0, e, std::min(e + kBT, P.nat_rowptr[r + 1])));
         for (int32_t q = P.rnl_ptr[r]; q < P.rnl_ptr[r + 1]; q += kBT)
             long_chunks.push_back(make_int4(r, 1, q, std::min(q + kBT, P.rnl_ptr[r + 1])));
+        for (int32_t e = P.nat_rowptr[r]; e < P.nat_rowptr[r + 1]; e++) {
+            const int32_t t0 = P.jnl_ptr[e], t1 = P.jnl_ptr[e + 1];
+            if (t1 - t0 <= kHeavyRefs) continue;
+            for (int32_t t = t0; t < t1; t += kBT) long_chunks.push_back(make_int4(r, 2 + e, t, std::min(t + kBT, t1)));
+        }
         long_cptr.push_back((int32_t)long_chunks.size());
     }
     for (int32_t q = 0; q < P.n_mos; q++)
@@ -980,41 +1186,70 @@ int upload(BigCircuit& C, std::string& err) {
     // launch tables
     std::vector<int4> tiles, trsm, gemm;
     std::vector<int32_t> fronts, borig;
-    std::vector<int2> diag;
+    std::vector<int2> diag, cbr;
+    // the scatter gathers find a tile's block columns by bisection of the
+    // relative maps, which the symbolic analysis builds ascending
+    for (int c = 0; c < B.ns; c++)
+        for (int a = B.rel_ptr[c] + 1; a < B.rel_ptr[c + 1]; a++)
+            if (B.rel[a] <= B.rel[a - 1]) {
+                err = "supernodal plan: relative map not ascending";
+                return CS_E_INVALID;
+            }
     const int nlev = (int)B.lev_ptr.size() - 1;
     C.levels.resize(nlev);
     for (int l = 0; l < nlev; l++) {
         auto& Lv = C.levels[l];
         Lv.tile_b = (int)tiles.size();
-        int small_max = 0, m_max = 0;
-        std::vector<int32_t> small, large;
+        int small_max[kSmallBuckets] = {}, m_max = 0;
+        std::vector<int32_t> small[kSmallBuckets], large;
         Lv.orig_b = (int)borig.size();
         for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) {
             const int s = B.lev_sn[q], m = B.sn_m[s];
             if (m <= kSmallM) {
-                small.push_back(s);
-                small_max = std::max(small_max, m);
+                int bk = 0;
+                while (m > kBucketMax[bk]) bk++;
+                small[bk].push_back(s);
+                small_max[bk] = std::max(small_max[bk], m);
             } else {
                 large.push_back(s);
                 const int step = std::max(1, kGatherChunk / std::max(m, 1));
-                for (int j = 0; j < m; j += step) tiles.push_back(make_int4(s, j, std::min(m, j + step), 0));
+                for (int j = 0; j < m; j += step) {
+                    const int j1 = std::min(m, j + step);
+                    tiles.push_back(make_int4(s, j, j1, (int)cbr.size()));
+                    for (int cq = B.ch_ptr[s]; cq < B.ch_ptr[s + 1]; cq++) {
+                        const int c = B.ch_sn[cq];
+                        const int32_t* rl = B.rel.data() + B.rel_ptr[c];
+                        const int rc = B.rel_ptr[c + 1] - B.rel_ptr[c];
+                        cbr.push_back(make_int2((int)(std::lower_bound(rl, rl + rc, j) - rl),
+                                                (int)(std::lower_bound(rl, rl + rc, j1) - rl)));
+                    }
+                }
                 borig.insert(borig.end(), B.osn_pos.begin() + B.osn_ptr[s], B.osn_pos.begin() + B.osn_ptr[s + 1]);
             }
             m_max = std::max(m_max, m);
         }
         Lv.orig_e = (int)borig.size();
         Lv.tile_n = (int)tiles.size() - Lv.tile_b;
-        Lv.small_b = (int)fronts.size();
-        fronts.insert(fronts.end(), small.begin(), small.end());
-        Lv.small_n = (int)small.size();
-        Lv.small_lds = sizeof(double) * (size_t)small_max * small_max;
+        for (int bk = 0; bk < kSmallBuckets; bk++) {
+            Lv.small_b[bk] = (int)fronts.size();
+            fronts.insert(fronts.end(), small[bk].begin(), small[bk].end());
+            Lv.small_n[bk] = (int)small[bk].size();
+            Lv.small_lds[bk] = sizeof(double) * (size_t)small_max[bk] * small_max[bk];
+        }
         Lv.fr_b = (int)fronts.size();
         for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) fronts.push_back(B.lev_sn[q]);
         Lv.fr_n = (int)fronts.size() - Lv.fr_b;
         Lv.big_b = (int)fronts.size();
         fronts.insert(fronts.end(), large.begin(), large.end());
         Lv.big_n = (int)large.size();
-        Lv.solve_lds = sizeof(double) * (size_t)std::max(m_max, 1);
+        int k_lev = 0;
+        for (int q = B.lev_ptr[l]; q < B.lev_ptr[l + 1]; q++) {
+            const int sq = B.lev_sn[q];
+            k_lev = std::max(k_lev, B.sn_first[sq + 1] - B.sn_first[sq]);
+        }
+        // solves: the front's vector, then its diagonal block (up to 64 x 65)
+        Lv.ld_off = (std::max(m_max, 1) + 1) & ~1;
+        Lv.solve_lds = sizeof(double) * ((size_t)Lv.ld_off + (size_t)std::min(k_lev, 64) * 65);
         int kmax = 0;
         for (int s : large) kmax = std::max(kmax, B.sn_first[s + 1] - B.sn_first[s]);
         for (int p0 = 0; p0 < kmax; p0 += kNb) {
@@ -1040,7 +1275,7 @@ int upload(BigCircuit& C, std::string& err) {
     }
     std::vector<char> tb;
     size_t o_t = pack_add(tb, tiles), o_f = pack_add(tb, fronts), o_d = pack_add(tb, diag),
-           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm), o_bo = pack_add(tb, borig);
+           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm), o_bo = pack_add(tb, borig), o_cb = pack_add(tb, cbr);
     BCHK(hipMalloc(&C.tbuf, tb.size()));
     BCHK(hipMemcpy(C.tbuf, tb.data(), tb.size(), hipMemcpyHostToDevice));
     char* t = (char*)C.tbuf;
@@ -1050,6 +1285,7 @@ int upload(BigCircuit& C, std::string& err) {
     C.d_trsm = (int4*)(t + o_tr);
     C.d_gemm = (int4*)(t + o_g);
     C.d_borig = (int32_t*)(t + o_bo);
+    C.d_cbr = (int2*)(t + o_cb);
 
     // workspace
     const size_t n = B.n, nnz = P.nnz_nat;
@@ -1097,28 +1333,40 @@ int upload(BigCircuit& C, std::string& err) {
 void launch_level(BigCircuit& C, hipStream_t s, int l, bool fused) {
     const BigDev* P = C.d_dev;
     auto& Lv = C.levels[l];
-    if (Lv.small_n)
-        hipLaunchKernelGGL(k_mf_lvl_small, dim3(Lv.small_n), dim3(kBT), Lv.small_lds, s, P, C.pool, C.dinv, C.Aval,
-                           C.d_fronts + Lv.small_b);
+    if (Lv.small_n[0])
+        MF_LAUNCH(CS_MFK_SMALL32, l, (k_mf_lvl_small<8, 4>), dim3(Lv.small_n[0]), dim3(64), Lv.small_lds[0], s, P,
+                  C.pool, C.dinv, C.Aval, C.d_fronts + Lv.small_b[0]);
+    if (Lv.small_n[1])
+        MF_LAUNCH(CS_MFK_SMALL64, l, (k_mf_lvl_small<16, 4>), dim3(Lv.small_n[1]), dim3(256), Lv.small_lds[1], s, P,
+                  C.pool, C.dinv, C.Aval, C.d_fronts + Lv.small_b[1]);
+    if (Lv.small_n[2])
+        MF_LAUNCH(CS_MFK_SMALL96, l, (k_mf_lvl_small<16, 6>), dim3(Lv.small_n[2]), dim3(256), Lv.small_lds[2], s, P,
+                  C.pool, C.dinv, C.Aval, C.d_fronts + Lv.small_b[2]);
     if (!Lv.big_n) return;
     if (fused) {
-        hipLaunchKernelGGL(k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
-                           C.d_fronts + Lv.big_b);
+        MF_LAUNCH(CS_MFK_FRONT, l, k_mf_lvl_big, dim3(Lv.big_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.Aval,
+                  C.d_fronts + Lv.big_b);
         return;
     }
-    if (Lv.tile_n) hipLaunchKernelGGL(k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b);
+    if (Lv.tile_n)
+        MF_LAUNCH(CS_MFK_GATHER, l, k_mf_gather, dim3(Lv.tile_n), dim3(kBT), 0, s, P, C.pool, C.d_tiles + Lv.tile_b,
+                  C.d_cbr);
     if (Lv.orig_e > Lv.orig_b)
-        hipLaunchKernelGGL(k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval, C.pool,
-                           C.d_borig, Lv.orig_b, Lv.orig_e);
+        MF_LAUNCH(CS_MFK_ORIG, l, k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval,
+                  C.pool, C.d_borig, Lv.orig_b, Lv.orig_e);
     for (auto& pn : Lv.panels) {
-        hipLaunchKernelGGL(k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_diag + pn.diag_b);
+        MF_LAUNCH(CS_MFK_DIAG, l, k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
+                  C.d_diag + pn.diag_b);
         if (pn.trsm_n)
-            hipLaunchKernelGGL(k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv, C.d_trsm + pn.trsm_b);
+            MF_LAUNCH(CS_MFK_TRSM, l, k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
+                      C.d_trsm + pn.trsm_b);
         if (pn.gemm_n) {
             if (C.mfma)
-                hipLaunchKernelGGL(k_mf_gemm_mfma, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+                MF_LAUNCH(CS_MFK_GEMM, l, k_mf_gemm_mfma, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool,
+                          C.d_gemm + pn.gemm_b);
             else
-                hipLaunchKernelGGL(k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool, C.d_gemm + pn.gemm_b);
+                MF_LAUNCH(CS_MFK_GEMM, l, k_mf_gemm, dim3(pn.gemm_n), dim3(kBT), 0, s, P, C.pool,
+                          C.d_gemm + pn.gemm_b);
         }
     }
 }
@@ -1166,12 +1414,17 @@ int tune_levels(BigCircuit& C, hipStream_t s, std::string& err) {
 
 void launch_solve(BigCircuit& C, hipStream_t s) {
     const BigDev* P = C.d_dev;
-    for (auto& Lv : C.levels)
-        hipLaunchKernelGGL(k_mf_fwd, dim3(Lv.fr_n), dim3(kBT), Lv.solve_lds, s, P, C.pool, C.x, C.uvec,
-                           C.d_fronts + Lv.fr_b);
-    for (auto it = C.levels.rbegin(); it != C.levels.rend(); ++it)
-        hipLaunchKernelGGL(k_mf_bwd, dim3(it->fr_n), dim3(kBT), it->solve_lds, s, P, C.pool, C.dinv, C.x,
-                           C.d_fronts + it->fr_b);
+    const int nl = (int)C.levels.size();
+    for (int l = 0; l < nl; l++) {
+        auto& Lv = C.levels[l];
+        MF_LAUNCH(CS_MFK_FWD, l, k_mf_fwd, dim3(Lv.fr_n), dim3(kBT), Lv.solve_lds, s, P, C.pool, C.x, C.uvec,
+                  C.d_fronts + Lv.fr_b, Lv.ld_off);
+    }
+    for (int l = nl - 1; l >= 0; l--) {
+        auto& Lv = C.levels[l];
+        MF_LAUNCH(CS_MFK_BWD, l, k_mf_bwd, dim3(Lv.fr_n), dim3(kBT), Lv.solve_lds, s, P, C.pool, C.dinv, C.x,
+                  C.d_fronts + Lv.fr_b, Lv.ld_off);
+    }
 }
 
 // Jacobian and residual: short rows one lane each, long rows one workgroup
@@ -1184,8 +1437,8 @@ int launch_assemble(BigCircuit& C, hipStream_t s, const double* qx, const double
     if (C.hd.n_long) {
         hipLaunchKernelGGL(k_assemble_chunks, dim3(C.hd.n_chunks), dim3(kBT), 0, s, P, C.Glin, C.Clin, C.sp, C.u,
                            C.st, pin, C.Aval, C.lpart);
-        hipLaunchKernelGGL(k_long_finish, dim3((C.hd.n_long + 63) / 64), dim3(64), 0, s, P, C.sp, C.u, qx, pin,
-                           C.lpart, x, part ? part + gr : nullptr);
+        hipLaunchKernelGGL(k_long_finish, dim3(C.hd.n_long), dim3(kBT), 0, s, P, C.sp, C.u, qx, pin, C.lpart,
+                           C.Aval, x, part ? part + gr : nullptr);
     }
     return gr + C.hd.n_long;
 }
@@ -1194,7 +1447,7 @@ void launch_charges(BigCircuit& C, hipStream_t s, const double* v, double* q) {
     hipLaunchKernelGGL(k_charges, dim3(grid_for(C.B.n)), dim3(kBT), 0, s, C.d_dev, C.Clin, v, q);
     if (C.hd.n_long) {
         hipLaunchKernelGGL(k_charges_chunks, dim3(C.hd.n_chunks), dim3(kBT), 0, s, C.d_dev, C.Clin, v, C.lpart);
-        hipLaunchKernelGGL(k_charges_finish, dim3((C.hd.n_long + 63) / 64), dim3(64), 0, s, C.d_dev, C.lpart, q);
+        hipLaunchKernelGGL(k_charges_finish, dim3(C.hd.n_long), dim3(kBT), 0, s, C.d_dev, C.lpart, q);
     }
 }
 
@@ -1607,6 +1860,43 @@ int big_lu_solve(BigCircuit* C, hipStream_t s, double* x, std::string& err) {
     BCHK(hipEventElapsedTime(&ms, C->ev[2], C->ev[3]));
     C->solve_ms = ms;
     return CS_OK;
+}
+
+int big_profile(BigCircuit* C, hipStream_t s, int32_t* kinds, int32_t* levels, float* ms, int32_t cap,
+                int32_t* count, std::string& err) {
+    if (!C->tuned) {
+        int rc = tune_levels(*C, s, err);
+        if (rc) return rc;
+    }
+    BigCircuit::LaunchProf P;
+    C->prof = &P;
+    launch_factor(*C, s);
+    launch_solve(*C, s);
+    C->prof = nullptr;
+    hipError_t e = hipStreamSynchronize(s);
+    const int nlaunch = (int)P.kind.size();
+    int rc = CS_OK;
+    if (e != hipSuccess || (int)P.ev.size() != 2 * nlaunch) {
+        err = e != hipSuccess ? std::string("hipStreamSynchronize: ") + hipGetErrorString(e)
+                              : std::string("event creation failed");
+        rc = CS_E_HIP;
+    }
+    for (int q = 0; rc == CS_OK && q < nlaunch; q++) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, P.ev[2 * q], P.ev[2 * q + 1]) != hipSuccess) {
+            err = "hipEventElapsedTime failed";
+            rc = CS_E_HIP;
+            break;
+        }
+        if (q < cap) {
+            if (kinds) kinds[q] = P.kind[q];
+            if (levels) levels[q] = P.level[q];
+            if (ms) ms[q] = t;
+        }
+    }
+    for (auto ev : P.ev) (void)hipEventDestroy(ev);
+    if (count) *count = nlaunch;
+    return rc;
 }
 
 }  // namespace cedar

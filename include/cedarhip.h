@@ -220,6 +220,26 @@ int cs_big_info_get(cs_ctx* ctx, int32_t cid, cs_big_info* info);
 /* Device time of the last cs_lu_refactor and cs_lu_solve of a supernodal
  * circuit, milliseconds. */
 int cs_big_times(cs_ctx* ctx, int32_t cid, double* factor_ms, double* solve_ms);
+/* Per-launch timing of one factorization of the current Jacobian values
+ * and one forward/backward solve, as plain stream launches, each between
+ * two HIP events on the launch stream: for launch q, kinds[q] (CS_MFK_*),
+ * levels[q] (assembly-tree level) and ms[q]; *count = number of launches
+ * (at most cap are stored).  A diagnostic of the supernodal path with no
+ * counterpart in the reference (its KLU/UMFPACK calls are host library
+ * calls); it leaves the circuit's state as a factorization would. */
+#define CS_MFK_SMALL32 0   /* fronts up to 32 rows, gathered and factored in one kernel */
+#define CS_MFK_SMALL64 1   /* fronts of 33..64 rows                                     */
+#define CS_MFK_SMALL96 2   /* fronts of 65..96 rows                                     */
+#define CS_MFK_GATHER  3   /* contribution-block gather of the large fronts            */
+#define CS_MFK_ORIG    4   /* Jacobian entries into the large fronts                   */
+#define CS_MFK_DIAG    5   /* LU of a panel's diagonal block                           */
+#define CS_MFK_TRSM    6   /* a panel's row and column triangular solves               */
+#define CS_MFK_GEMM    7   /* a panel's trailing update                                */
+#define CS_MFK_FRONT   8   /* one workgroup per large front (the fused form)          */
+#define CS_MFK_FWD     9   /* forward substitution of one level                        */
+#define CS_MFK_BWD     10  /* backward substitution of one level                       */
+int cs_big_profile(cs_ctx* ctx, int32_t cid, int32_t* kinds, int32_t* levels, float* ms, int32_t cap,
+                   int32_t* count);
 
 /* ---- circuits (host symbolic analysis, done once per topology) --------- */
 /* Lowers the netlist, builds the Jacobian pattern, the slot maps for
