@@ -124,6 +124,23 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
         upd.push_back(a);
         upd.push_back(b);
     };
+    // X tasks of one level: tasks with more than kLongTask updates (a supply
+    // row, one entry per device on it) go last, flagged 2, for the whole
+    // wavefront to work through together
+    std::vector<int32_t> longx;
+    auto push_x = [&](int32_t k, int32_t ub, int32_t ue, int32_t fl) {
+        if (ue - ub > kLongTask) longx.insert(longx.end(), {k, ub, ue, fl | 2});
+        else xtask.insert(xtask.end(), {k, ub, ue, fl});
+    };
+    // appends the level's long tasks; returns the level's end index, with
+    // bit 15 set when the level has long tasks (descriptor end fields only)
+    auto end_level = [&]() {
+        const bool any = !longx.empty();
+        xtask.insert(xtask.end(), longx.begin(), longx.end());
+        longx.clear();
+        return (int32_t)xtask.size() / 4 | (any ? 0x8000 : 0);
+    };
+    auto idx = [](int32_t v) { return v & 0x7FFF; };
     for (int32_t l = 0; l < nA; l++) {
         if (l < (int32_t)P.lu_lptr.size() - 1)
             for (int32_t q = P.lu_lptr[l]; q < P.lu_lptr[l + 1]; q++) {
@@ -143,20 +160,20 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
             const int32_t ub = (int32_t)upd.size() / 2;
             for (int32_t e = P.rowptr[k]; e < P.diag[k]; e++) push_upd(e, P.colperm[e]);
             const int32_t ue = (int32_t)upd.size() / 2;
-            xtask.insert(xtask.end(), {k, ub, ue, 0});
+            push_x(k, ub, ue, 0);
         }
-        axt[l + 1] = (int32_t)xtask.size() / 4;
+        axt[l + 1] = end_level();
     }
-    bxt[0] = (int32_t)xtask.size() / 4;
+    bxt[0] = (int32_t)xtask.size() / 4;   // (the A levels' last end, without its flag)
     for (int32_t l = 0; l < nB; l++) {
         for (int32_t k = 0; k < n; k++) {
             if (bl[k] != l) continue;
             const int32_t ub = (int32_t)upd.size() / 2;
             for (int32_t e = P.diag[k] + 1; e < P.rowptr[k + 1]; e++) push_upd(e, P.colperm[e]);
             const int32_t ue = (int32_t)upd.size() / 2;
-            xtask.insert(xtask.end(), {k, ub, ue, 1});
+            push_x(k, ub, ue, 1);
         }
-        bxt[l + 1] = (int32_t)xtask.size() / 4;
+        bxt[l + 1] = end_level();
     }
     // transposed solves (adjoint): U^T z = b over U's columns, ascending
     // (scaled by 1/U_kk), then L^T w = z over L's columns, descending
@@ -189,9 +206,9 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
             if (tu[k] != l) continue;
             const int32_t ub = (int32_t)upd.size() / 2;
             for (auto& ie : ucol[k]) push_upd(ie.second, ie.first);
-            xtask.insert(xtask.end(), {k, ub, (int32_t)upd.size() / 2, 1});
+            push_x(k, ub, (int32_t)upd.size() / 2, 1);
         }
-        tudesc[2 * l + 1] = (int32_t)xtask.size() / 4;
+        tudesc[2 * l + 1] = end_level();
     }
     for (int32_t l = 0; l < nTL; l++) {
         tldesc[2 * l] = (int32_t)xtask.size() / 4;
@@ -199,12 +216,16 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
             if (tl[k] != l) continue;
             const int32_t ub = (int32_t)upd.size() / 2;
             for (auto& ie : lcolv[k]) push_upd(ie.second, ie.first);
-            xtask.insert(xtask.end(), {k, ub, (int32_t)upd.size() / 2, 0});
+            push_x(k, ub, (int32_t)upd.size() / 2, 0);
         }
-        tldesc[2 * l + 1] = (int32_t)xtask.size() / 4;
+        tldesc[2 * l + 1] = end_level();
     }
     if (upd.size() / 2 >= 0xFFFF) {
         *why = "too many LU updates for 16-bit indices";
+        return false;
+    }
+    if (xtask.size() / 4 >= 0x8000) {   // level ends carry a flag in bit 15
+        *why = "too many substitution tasks for 15-bit indices";
         return false;
     }
     // sources: data and dc value copied to wd
@@ -216,20 +237,36 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
     }
 
     std::vector<int32_t> rdesc(8 * (size_t)n), adesc(4 * (size_t)nA), bdesc(2 * (size_t)nB);
+    // long rows (more than kLongRow entries and references: the supply
+    // node's row) are assembled by the whole wavefront; their heavy entries
+    // (more than kHeavyRefs references) are listed per row
+    std::vector<int32_t> lrows, heavy;
     for (int32_t k = 0; k < n; k++) {
-        const int32_t e0 = P.rowptr[k];
-        const int32_t d[8] = {e0, P.rowptr[k + 1], P.diag[k], jptr[e0],
+        const int32_t e0 = P.rowptr[k], e1 = P.rowptr[k + 1];
+        const int32_t w = (e1 - e0) + (jptr[e1] - jptr[e0]) + (lptr[k + 1] - lptr[k]) + (rptr[k + 1] - rptr[k]);
+        const bool lng = w > kLongRow;
+        if (lng) {
+            if (nnz >= 0x8000) {   // the diagonal field carries the flag in bit 15
+                *why = "long row in a pattern too large for 15-bit entry indices";
+                return false;
+            }
+            const int32_t hb = (int32_t)heavy.size();
+            for (int32_t e = e0; e < e1; e++)
+                if (jptr[e + 1] - jptr[e] > kHeavyRefs) heavy.push_back(e);
+            lrows.insert(lrows.end(), {k, hb, (int32_t)heavy.size(), 0});
+        }
+        const int32_t d[8] = {e0, e1, P.diag[k] | (lng ? 0x8000 : 0), jptr[e0],
                               lptr[k], lptr[k + 1], rptr[k], rptr[k + 1]};
         for (int q = 0; q < 8; q++) rdesc[8 * k + q] = d[q];
     }
     for (int32_t l = 0; l < nA; l++) {
         adesc[4 * l] = aft[l];
         adesc[4 * l + 1] = aft[l + 1];
-        adesc[4 * l + 2] = axt[l];
+        adesc[4 * l + 2] = idx(axt[l]);
         adesc[4 * l + 3] = axt[l + 1];
     }
     for (int32_t l = 0; l < nB; l++) {
-        bdesc[2 * l] = bxt[l];
+        bdesc[2 * l] = idx(bxt[l]);
         bdesc[2 * l + 1] = bxt[l + 1];
     }
     Blob b;
@@ -260,6 +297,9 @@ bool build_compact(const Plan& P, std::vector<uint16_t>& blob, CPlan& cp,
             cdesc[2 * j + 1] = (int32_t)lcl.size();
         }
     }
+    cp.o_lrows = b.add(lrows, 4);
+    cp.o_heavy = b.add(heavy);
+    cp.n_lrows = (int32_t)lrows.size() / 4;
     cp.o_tudesc = b.add(tudesc, 2);
     cp.o_tldesc = b.add(tldesc, 2);
     cp.o_cdesc = b.add(cdesc, 2);

@@ -8,9 +8,12 @@
 // row-wise charge of row k belongs to the lane that owns k.
 //
 // Blob arrays (offsets in uint16 units, see CPlan):
-//   rdesc[8*n]   per pivot row k: e0, e1 (its F entries), diag, jq0 (first
-//                stamp ref of entry e0), l0, l1 (linear entries), r0, r1
-//                (residual stamp refs) — one 16-byte LDS read per row
+//   rdesc[8*n]   per pivot row k: e0, e1 (its F entries), diag (bit 15:
+//                a long row), jq0 (first stamp ref of entry e0), l0, l1
+//                (linear entries), r0, r1 (residual stamp refs) — one
+//                16-byte LDS read per row
+//   lrows[4*]    per long row: k, [hb, he) its heavy entries in heavy[], 0
+//   heavy[]      entry indices
 //   jptr[nnz+1] jref[]              nonlinear stamp refs per F entry
 //   rref[]                          nonlinear stamp refs per row (residual)
 //   lpos[] lcol[]                   linear entries: F position, column
@@ -27,7 +30,10 @@
 //                                   ctl < 0x7FFF: then *= inv[ctl] (L entry);
 //                                   0x7FFF: plain U entry; 0x8000|k: diagonal
 //                                   of row k, also inv[k] = 1/F[e]
-//   xtask[4*]  (k, ub, ue, fl)      x[k] -= sum F[a]x[b]; fl & 1: x[k] *= inv[k]
+//   xtask[4*]  (k, ub, ue, fl)      x[k] -= sum F[a]x[b]; fl & 1: x[k] *= inv[k];
+//                                   fl & 2: a long task, placed at the end of
+//                                   its level (whose end index then carries
+//                                   bit 15) and run by the whole wavefront
 //   upd[2*]    (a, b)
 // A levels run the LU and the forward substitution together (Ly = b is the
 // extra column of the factorisation); B levels run the backward
@@ -43,6 +49,9 @@
 
 namespace cedar {
 
+constexpr int32_t kLongRow = 48;     // a row with more entries + references is a long row
+constexpr int32_t kHeavyRefs = 8;    // an entry of a long row with more references is heavy
+constexpr int32_t kLongTask = 12;    // an X task with more updates is a long task
 constexpr uint32_t CP_PLAIN_U = 0x7FFF;
 constexpr uint32_t CP_DIAG = 0x8000;
 constexpr uint32_t CP_GROUND = 0xFFFF;
@@ -55,6 +64,7 @@ struct CPlan {
     int32_t o_rdesc, o_jptr, o_jref, o_rref, o_lpos, o_lcol;
     int32_t o_adesc, o_bdesc, o_src, o_ftask, o_xtask, o_upd;
     int32_t o_tudesc, o_tldesc, o_cdesc, o_lcl, o_lrow, nTU, nTL;
+    int32_t o_lrows, o_heavy, n_lrows;
     int32_t n_ftask, n_xtask, n_upd;
     // global arrays, read when an instance starts
     const uint16_t* blob;

@@ -45,6 +45,32 @@ static __device__ __noinline__ double cold_accept(double err, int k) { return ac
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
 
+__device__ __forceinline__ double sv(const double* st, uint32_t r) {
+    const double v = st[r >> 1];
+    return (r & 1) ? -v : v;
+}
+
+// Value of lane src (uniform) for the whole wavefront.
+__device__ __forceinline__ double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sequential sum, in list order, of the stamp references refs[q0, q1):
+// the lanes fetch 64 terms at a time and the sum runs over them in order,
+// so the result equals the one-lane loop bit for bit.  Uniform result.
+__device__ __forceinline__ double seq_ref_sum(double s, const double* st, const uint16_t* refs, int q0, int q1) {
+    for (int b = q0; b < q1; b += W) {
+        const int q = b + lane_id();
+        const double t = q < q1 ? sv(st, refs[q]) : 0.0;
+        const int cnt = (q1 - b) < W ? q1 - b : W;
+        for (int j = 0; j < cnt; j++) s += bcast(t, j);
+    }
+    return s;
+}
+
 // Synchronisation of the instance's own wavefront.  An instance is owned by
 // one wavefront, so its phases need only wavefront-scope ordering of LDS
 // accesses (other wavefronts of the workgroup own other instances): a fence
@@ -54,11 +80,6 @@ __device__ __forceinline__ void lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ double sv(const double* st, uint32_t r) {
-    const double v = st[r >> 1];
-    return (r & 1) ? -v : v;
 }
 
 // Per-lane MOSFET constants of NM device slots.
@@ -142,6 +163,11 @@ __device__ __forceinline__ void eval_sources(const CPlan& c, const View& v, doub
 
 // F = G + a0*C + nonlinear stamps; x = -residual (qx added per row).  With
 // pin, rows that carry charge become u_k = pin_k.  Returns max |residual|.
+// A lane assembles its own rows; long rows (flag in the diagonal field) are
+// assembled afterwards by the whole wavefront with the same operations in
+// the same order: short entries one lane each, heavy entries and the
+// residual's reference list as sequential sums over terms fetched 64 at a
+// time.
 template <int NR>
 __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double a0,
                                            const double (&qx)[NR], bool pin,
@@ -153,7 +179,8 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
         const int k = lane_id() + W * r;
         if (k < c.n) {
             const uint4 rd = reinterpret_cast<const uint4*>(B + c.o_rdesc)[k];
-            const int e0 = rd.x & 0xFFFF, e1 = rd.x >> 16, dg = rd.y & 0xFFFF;
+            if (rd.y & 0x8000) continue;   // a long row
+            const int e0 = rd.x & 0xFFFF, e1 = rd.x >> 16, dg = rd.y & 0x7FFF;
             double res;
             if (pin && (flag[r] & 1)) {
                 for (int e = e0; e < e1; e++) v.F[e] = (e == dg) ? 1.0 : 0.0;
@@ -183,12 +210,75 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
             if (a != a || a > lmax) lmax = a;
         }
     }
+    for (int i = 0; i < c.n_lrows; i++) {
+        const uint2 lr = reinterpret_cast<const uint2*>(B + c.o_lrows)[i];
+        const int k = lr.x & 0xFFFF, hb = lr.x >> 16, he = lr.y & 0xFFFF;
+        const uint4 rd = reinterpret_cast<const uint4*>(B + c.o_rdesc)[k];
+        const int e0 = rd.x & 0xFFFF, e1 = rd.x >> 16, dg = rd.y & 0x7FFF;
+        // the owner lane's per-row registers
+        const int owner = k % W, rk = k / W;
+        double qxo = 0.0, pio = 0.0;
+        uint32_t flo = 0;
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (r == rk) {
+                qxo = qx[r];
+                pio = pinv[r];
+                flo = flag[r];
+            }
+        qxo = bcast(qxo, owner);
+        pio = bcast(pio, owner);
+        flo = (uint32_t)__builtin_amdgcn_readlane((int)flo, owner);
+        double res;
+        if (pin && (flo & 1)) {
+            for (int e = e0 + lane_id(); e < e1; e += W) v.F[e] = (e == dg) ? 1.0 : 0.0;
+            res = v.u[k] - pio;
+        } else {
+            for (int e = e0 + lane_id(); e < e1; e += W) {
+                const int q0 = B[c.o_jptr + e], q1 = B[c.o_jptr + e + 1];
+                if (q1 - q0 > kHeavyRefs) continue;
+                double s = 0.0;
+                for (int q = q0; q < q1; q++) s += sv(v.st, B[c.o_jref + q]);
+                v.F[e] = s;
+            }
+            for (int h = hb; h < he; h++) {
+                const int e = B[c.o_heavy + h];
+                const double s = seq_ref_sum(0.0, v.st, B + c.o_jref, B[c.o_jptr + e], B[c.o_jptr + e + 1]);
+                if (lane_id() == 0) v.F[e] = s;
+            }
+            lds_sync();
+            // linear entries: F updates in parallel (distinct entries), the
+            // residual's products summed in order
+            double acc = 0.0;
+            const int l0 = rd.z & 0xFFFF, l1 = rd.z >> 16;
+            for (int b = l0; b < l1; b += W) {
+                const int l = b + lane_id();
+                double jl = 0.0, ul = 0.0;
+                if (l < l1) {
+                    jl = v.G[l] + a0 * v.C[l];
+                    v.F[B[c.o_lpos + l]] += jl;
+                    ul = v.u[B[c.o_lcol + l]];
+                }
+                const int cnt = (l1 - b) < W ? l1 - b : W;
+                for (int j = 0; j < cnt; j++) acc += bcast(jl, j) * bcast(ul, j);
+            }
+            acc += qxo;
+            res = seq_ref_sum(acc, v.st, B + c.o_rref, rd.w & 0xFFFF, rd.w >> 16);
+        }
+        if (lane_id() == 0) {
+            v.x[k] = -res;
+            const double a = fabs(res);
+            if (a != a || a > lmax) lmax = a;
+        }
+    }
     return lmax;
 }
 
-// One X task: x[k] -= sum F[a] x[b]; scaled by inv[k] when fl & 1.
+// One X task: x[k] -= sum F[a] x[b]; scaled by inv[k] when fl & 1.  Long
+// tasks (fl & 2) are skipped here and run by x_task_long.
 __device__ __forceinline__ void x_task(const uint2* xt, const uint32_t* up, int q, const View& v) {
     const uint2 t = xt[q];
+    if ((t.y >> 16) & 2) return;
     const int k = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
     double acc = v.x[k];
     for (int w = ub; w < ue; w++) {
@@ -198,6 +288,38 @@ __device__ __forceinline__ void x_task(const uint2* xt, const uint32_t* up, int 
     v.x[k] = (t.y >> 16) & 1 ? acc * v.inv[k] : acc;
 }
 
+// A long X task by the whole wavefront: the lanes fetch 64 (F, x) pairs at
+// a time and the multiply-subtracts run over them in order (the same
+// operations as x_task).
+__device__ __forceinline__ void x_task_long(const uint2 t, const uint32_t* up, const View& v) {
+    const int k = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
+    double acc = v.x[k];
+    for (int b = ub; b < ue; b += W) {
+        const int w = b + lane_id();
+        double f = 0.0, xb = 0.0;
+        if (w < ue) {
+            const uint32_t ab = up[w];
+            f = v.F[ab & 0xFFFF];
+            xb = v.x[ab >> 16];
+        }
+        const int cnt = (ue - b) < W ? ue - b : W;
+        for (int j = 0; j < cnt; j++) acc -= bcast(f, j) * bcast(xb, j);
+    }
+    if (lane_id() == 0) v.x[k] = (t.y >> 16) & 1 ? acc * v.inv[k] : acc;
+}
+
+// The long tasks of a level sit at the end of its range [xb, xe); a level
+// that has them carries bit 15 in its end index.
+__device__ __forceinline__ void x_long_tasks(const uint2* xt, const uint32_t* up, int xb, int xe_flag,
+                                             const View& v) {
+    if (!(xe_flag & 0x8000)) return;
+    for (int q = (xe_flag & 0x7FFF) - 1; q >= xb; q--) {
+        const uint2 t = xt[q];
+        if (!((t.y >> 16) & 2)) break;
+        x_task_long(t, up, v);
+    }
+}
+
 // X-task levels given by (begin, end) descriptors.
 __device__ __forceinline__ void x_levels(const CPlan& c, const View& v, int o_desc, int nlev) {
     const uint16_t* B = v.B;
@@ -205,8 +327,9 @@ __device__ __forceinline__ void x_levels(const CPlan& c, const View& v, int o_de
     const uint32_t* up = reinterpret_cast<const uint32_t*>(B + c.o_upd);
     for (int l = 0; l < nlev; l++) {
         const uint32_t bd = reinterpret_cast<const uint32_t*>(B + o_desc)[l];
-        const int xb = bd & 0xFFFF, nx = (bd >> 16) - xb;
+        const int xb = bd & 0xFFFF, xe = bd >> 16, nx = (xe & 0x7FFF) - xb;
         for (int q = lane_id(); q < nx; q += W) x_task(xt, up, xb + q, v);
+        x_long_tasks(xt, up, xb, xe, v);
         lds_sync();
     }
 }
@@ -223,7 +346,7 @@ __device__ __forceinline__ void factor(const CPlan& c, const View& v) {
     for (int l = 0; l < c.nA; l++) {
         const uint2 ad = reinterpret_cast<const uint2*>(B + c.o_adesc)[l];
         const int fb = ad.x & 0xFFFF, nf = (ad.x >> 16) - fb;
-        const int xb = ad.y & 0xFFFF, nx = FWD ? (int)(ad.y >> 16) - xb : 0;
+        const int xb = ad.y & 0xFFFF, nx = FWD ? (int)((ad.y >> 16) & 0x7FFF) - xb : 0;
         for (int q = lane_id(); q < nf + nx; q += W) {
             if (q < nf) {
                 const uint2 t = ft[fb + q];
@@ -244,6 +367,7 @@ __device__ __forceinline__ void factor(const CPlan& c, const View& v) {
                 x_task(xt, up, xb + q - nf, v);
             }
         }
+        if (FWD) x_long_tasks(xt, up, xb, (int)(ad.y >> 16), v);
         lds_sync();
     }
 }
