@@ -973,8 +973,6 @@ static int build_pack(cs_ctx* ctx, Batch& B, const std::vector<int32_t>& used,
     if ((rc = dalloc(&d_order, sizeof(int32_t) * order.size())) ||
         (rc = dalloc(&d_chunks, sizeof(int4) * chunks.size())) || (rc = dalloc(&d_q, sizeof(int32_t))))
         return rc;
-    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice));
     LdsClass& L = B.pack;
     L.nr = nr;
     L.lds = need;
@@ -982,6 +980,26 @@ static int build_pack(cs_ctx* ctx, Batch& B, const std::vector<int32_t>& used,
     const int per_cu = pack_max_blocks(nr, need);
     if (per_cu <= 0) return fail(ctx, CS_E_RANGE, "packed LDS kernel does not fit on a CU");
     L.nblocks = (int32_t)std::min<int64_t>((int64_t)chunks.size(), (int64_t)per_cu * std::max(ctx->n_cu, 1));
+    // When the chunks outnumber the resident workgroups by E, the E extra
+    // chunks start when the first workgroups finish.  The chunks are sorted
+    // by circuit size, largest first, and a larger circuit takes longer; so
+    // the queue puts the E smallest chunks first (ascending), then the large
+    // ones, then the next E smallest (descending): the workgroups that
+    // finish first take a second small chunk, the largest of those first,
+    // instead of a large chunk being followed by a small one.
+    {
+        const size_t C = chunks.size(), E = C - (size_t)L.nblocks;
+        if (E > 0 && 2 * E <= C) {
+            std::vector<int4> q;
+            q.reserve(C);
+            for (size_t i = 0; i < E; i++) q.push_back(chunks[C - 1 - i]);
+            for (size_t i = 0; i < C - 2 * E; i++) q.push_back(chunks[i]);
+            for (size_t i = C - 2 * E; i < C - E; i++) q.push_back(chunks[i]);
+            chunks.swap(q);
+        }
+    }
+    HIPCHK(ctx, hipMemcpy(d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice));
     LdsArgs& a = L.args;
     a = B.classes.empty() ? LdsArgs{} : B.classes[0].args;
     a.order = (const int32_t*)d_order;
