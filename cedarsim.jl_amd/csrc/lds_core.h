@@ -45,9 +45,23 @@ static __device__ __noinline__ double cold_accept(double err, int k) { return ac
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
 
+// Stamp reference r = (slot << 1) | negate: the sign goes straight into the
+// sign bit (negation is exactly a sign flip).
 __device__ __forceinline__ double sv(const double* st, uint32_t r) {
     const double v = st[r >> 1];
-    return (r & 1) ? -v : v;
+    return __longlong_as_double(__double_as_longlong(v) ^ ((long long)(r & 1u) << 63));
+}
+
+// Sum of the stamp references refs[q, qe) onto s, in order, two references
+// per step so that both loads are in flight together.
+__device__ __forceinline__ double ref_sum(double s, const double* st, const uint16_t* refs, int q, int qe) {
+    for (; q + 1 < qe; q += 2) {
+        const double t0 = sv(st, refs[q]), t1 = sv(st, refs[q + 1]);
+        s += t0;
+        s += t1;
+    }
+    if (q < qe) s += sv(st, refs[q]);
+    return s;
 }
 
 // Value of lane src (uniform) for the whole wavefront.
@@ -189,9 +203,8 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
                 int q = rd.y >> 16;
                 for (int e = e0; e < e1; e++) {
                     const int qe = B[c.o_jptr + e + 1];
-                    double s = 0.0;
-                    for (; q < qe; q++) s += sv(v.st, B[c.o_jref + q]);
-                    v.F[e] = s;
+                    v.F[e] = ref_sum(0.0, v.st, B + c.o_jref, q, qe);
+                    q = qe;
                 }
                 double acc = 0.0;
                 const int l0 = rd.z & 0xFFFF, l1 = rd.z >> 16;
@@ -201,9 +214,7 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
                     acc += jl * v.u[B[c.o_lcol + l]];
                 }
                 acc += qx[r];
-                const int r0 = rd.w & 0xFFFF, r1 = rd.w >> 16;
-                for (int q2 = r0; q2 < r1; q2++) acc += sv(v.st, B[c.o_rref + q2]);
-                res = acc;
+                res = ref_sum(acc, v.st, B + c.o_rref, rd.w & 0xFFFF, rd.w >> 16);
             }
             v.x[k] = -res;
             const double a = fabs(res);
@@ -237,9 +248,7 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
             for (int e = e0 + lane_id(); e < e1; e += W) {
                 const int q0 = B[c.o_jptr + e], q1 = B[c.o_jptr + e + 1];
                 if (q1 - q0 > kHeavyRefs) continue;
-                double s = 0.0;
-                for (int q = q0; q < q1; q++) s += sv(v.st, B[c.o_jref + q]);
-                v.F[e] = s;
+                v.F[e] = ref_sum(0.0, v.st, B + c.o_jref, q0, q1);
             }
             for (int h = hb; h < he; h++) {
                 const int e = B[c.o_heavy + h];
