@@ -122,10 +122,14 @@ def rc_mesh(nx: int = 450, ny: int = 450, n_inv: int = 10_000, seed: int = 0, vd
     neighbours, C ~ U[cmin, cmax] to ground at every node) driven by n_inv
     square-law inverters (the devices of ``inverter``) whose outputs sit on
     distinct random mesh nodes, each inverter with its own PULSE input
-    (0 -> vdd, rise/fall tr, period ``period``, 50 % duty) at a random phase
-    in [0, period/2) on a ``phase_step`` grid (the grid bounds the number of
-    distinct source breakpoints), VDD ``vdd``.  Built directly as a flat
-    netlist (mesh node (i, j) is unknown i*ny + j)."""
+    (0 -> vdd, rise/fall tr, period ``period``, 50 % duty) delayed by a
+    random td in [0, period/2 - tr] on a ``phase_step`` grid (the grid bounds
+    the number of distinct source breakpoints), VDD ``vdd``.  The bound keeps
+    each pulse inside its period: the reference evaluates PULSE on
+    ``t mod period`` (src/spectre_env.jl:153-166), so a pulse whose last
+    corner td + tr + pw + tf passed the period would be cut at the period
+    boundary, a jump in the source that no step size resolves.  Built
+    directly as a flat netlist (mesh node (i, j) is unknown i*ny + j)."""
     rng = np.random.default_rng(seed)
     nm = nx * ny
     if n_inv > nm:
@@ -142,9 +146,10 @@ def rc_mesh(nx: int = 450, ny: int = 450, n_inv: int = 10_000, seed: int = 0, vd
     in0 = nm + 1
     n_nodes = nm + 1 + n_inv
     out_nodes = rng.choice(nm, n_inv, replace=False).astype(np.int32)
-    nphase = max(1, int(round(0.5 * period / phase_step)))
-    phases = rng.integers(0, nphase, n_inv) * phase_step
     pw = 0.5 * period - tr
+    # td + tr + pw + tf <= period
+    nphase = max(1, int(np.floor((period - 2 * tr - pw) / phase_step + 1e-9)) + 1)
+    phases = rng.integers(0, nphase, n_inv) * phase_step
     # waves: 0 = VDD (DC), 1 + k = PULSE of inverter k
     wave_kind = np.concatenate([[WAVE_DC], np.full(n_inv, WAVE_PULSE)]).astype(np.int32)
     wave_len = np.concatenate([[1], np.full(n_inv, 7)]).astype(np.int32)

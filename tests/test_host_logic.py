@@ -165,3 +165,36 @@ def test_unimplemented_device_raises():
         Named(ch.UnimplementedDevice(), "U")(vcc, gnd)
     with pytest.raises(ch.CedarError):
         ch.lower(ERR)
+
+
+def test_power_grid_pulses_fit_their_period():
+    """Every PULSE of the power-grid workload ends inside its period: the
+    waveform is evaluated on t mod period, so a pulse whose last corner
+    passed the period would be cut there (a jump no step size resolves)."""
+    from cedarhip.workloads import rc_mesh
+    f = rc_mesh(20, 20, 300, seed=4)
+    a = f.arrays
+    kinds, off, data = a["wave_kind"], a["wave_off"], a["wave_data"]
+    n_pulse = 0
+    for k in range(len(kinds)):
+        if kinds[k] != 1:
+            continue
+        v1, v2, td, tr, tf, pw, per = data[off[k]:off[k] + 7]
+        assert td >= 0.0 and td + tr + pw + tf <= per * (1 + 1e-12)
+        n_pulse += 1
+    assert n_pulse == 300
+
+
+def test_power_grid_full_span_on_the_oracle():
+    """A small instance of the power-grid workload runs the configuration's
+    whole 0-5 ns span on the oracle (sparse LU); with pulses crossing their
+    period it stopped at t = 1 ns with DtLessThanMin."""
+    from cedarhip.workloads import rc_mesh
+    f = rc_mesh(24, 24, 40, seed=1)
+    o = ch.TranOptions(t_start=0.0, t_stop=5e-9, reltol=1e-6, abstol=1e-9)
+    O.set_sparse(True)
+    try:
+        w, uf, st, rc = O.tran(f, f.params(), None, o)
+    finally:
+        O.set_sparse(False)
+    assert rc == 0 and st.t_end == pytest.approx(5e-9, rel=1e-12)
