@@ -72,15 +72,20 @@ __device__ __forceinline__ double bcast(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Sequential sum, in list order, of the stamp references refs[q0, q1):
-// the lanes fetch 64 terms at a time and the sum runs over them in order,
-// so the result equals the one-lane loop bit for bit.  Uniform result.
-__device__ __forceinline__ double seq_ref_sum(double s, const double* st, const uint16_t* refs, int q0, int q1) {
+// Sum over the wavefront by a fixed butterfly (xor partners 32, 16, ..., 1):
+// every lane ends with the same value, the same on every run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    return v;
+}
+
+// s plus the stamp references refs[q0, q1), summed by the whole wavefront:
+// 64 terms at a time, each batch by wave_sum, the batches in order.
+__device__ __forceinline__ double wave_ref_sum(double s, const double* st, const uint16_t* refs, int q0, int q1) {
     for (int b = q0; b < q1; b += W) {
         const int q = b + lane_id();
-        const double t = q < q1 ? sv(st, refs[q]) : 0.0;
-        const int cnt = (q1 - b) < W ? q1 - b : W;
-        for (int j = 0; j < cnt; j++) s += bcast(t, j);
+        s += wave_sum(q < q1 ? sv(st, refs[q]) : 0.0);
     }
     return s;
 }
@@ -178,10 +183,10 @@ __device__ __forceinline__ void eval_sources(const CPlan& c, const View& v, doub
 // F = G + a0*C + nonlinear stamps; x = -residual (qx added per row).  With
 // pin, rows that carry charge become u_k = pin_k.  Returns max |residual|.
 // A lane assembles its own rows; long rows (flag in the diagonal field) are
-// assembled afterwards by the whole wavefront with the same operations in
-// the same order: short entries one lane each, heavy entries and the
-// residual's reference list as sequential sums over terms fetched 64 at a
-// time.
+// assembled afterwards by the whole wavefront: short entries one lane each,
+// heavy entries, the linear products and the residual's reference list as
+// wavefront sums of 64 terms at a time (a fixed summation order, so runs are
+// bit-identical; it differs from the one-lane order only by rounding).
 template <int NR>
 __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double a0,
                                            const double (&qx)[NR], bool pin,
@@ -252,27 +257,26 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
             }
             for (int h = hb; h < he; h++) {
                 const int e = B[c.o_heavy + h];
-                const double s = seq_ref_sum(0.0, v.st, B + c.o_jref, B[c.o_jptr + e], B[c.o_jptr + e + 1]);
+                const double s = wave_ref_sum(0.0, v.st, B + c.o_jref, B[c.o_jptr + e], B[c.o_jptr + e + 1]);
                 if (lane_id() == 0) v.F[e] = s;
             }
             lds_sync();
             // linear entries: F updates in parallel (distinct entries), the
-            // residual's products summed in order
+            // residual's products summed by the wavefront
             double acc = 0.0;
             const int l0 = rd.z & 0xFFFF, l1 = rd.z >> 16;
             for (int b = l0; b < l1; b += W) {
                 const int l = b + lane_id();
-                double jl = 0.0, ul = 0.0;
+                double p = 0.0;
                 if (l < l1) {
-                    jl = v.G[l] + a0 * v.C[l];
+                    const double jl = v.G[l] + a0 * v.C[l];
                     v.F[B[c.o_lpos + l]] += jl;
-                    ul = v.u[B[c.o_lcol + l]];
+                    p = jl * v.u[B[c.o_lcol + l]];
                 }
-                const int cnt = (l1 - b) < W ? l1 - b : W;
-                for (int j = 0; j < cnt; j++) acc += bcast(jl, j) * bcast(ul, j);
+                acc += wave_sum(p);
             }
             acc += qxo;
-            res = seq_ref_sum(acc, v.st, B + c.o_rref, rd.w & 0xFFFF, rd.w >> 16);
+            res = wave_ref_sum(acc, v.st, B + c.o_rref, rd.w & 0xFFFF, rd.w >> 16);
         }
         if (lane_id() == 0) {
             v.x[k] = -res;
@@ -297,22 +301,19 @@ __device__ __forceinline__ void x_task(const uint2* xt, const uint32_t* up, int 
     v.x[k] = (t.y >> 16) & 1 ? acc * v.inv[k] : acc;
 }
 
-// A long X task by the whole wavefront: the lanes fetch 64 (F, x) pairs at
-// a time and the multiply-subtracts run over them in order (the same
-// operations as x_task).
+// A long X task by the whole wavefront: x[k] minus the products F[a] x[b],
+// 64 at a time, each batch summed by wave_sum, the batches in order.
 __device__ __forceinline__ void x_task_long(const uint2 t, const uint32_t* up, const View& v) {
     const int k = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
     double acc = v.x[k];
     for (int b = ub; b < ue; b += W) {
         const int w = b + lane_id();
-        double f = 0.0, xb = 0.0;
+        double p = 0.0;
         if (w < ue) {
             const uint32_t ab = up[w];
-            f = v.F[ab & 0xFFFF];
-            xb = v.x[ab >> 16];
+            p = v.F[ab & 0xFFFF] * v.x[ab >> 16];
         }
-        const int cnt = (ue - b) < W ? ue - b : W;
-        for (int j = 0; j < cnt; j++) acc -= bcast(f, j) * bcast(xb, j);
+        acc -= wave_sum(p);
     }
     if (lane_id() == 0) v.x[k] = (t.y >> 16) & 1 ? acc * v.inv[k] : acc;
 }
