@@ -37,7 +37,7 @@ FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 # rocprofv3 --pmc passes, each scaled by 1.00 (a 1 GiB reduction reads 1.000
 # GiB on FETCH_SIZE, a 1 GiB fill writes 1.000 GiB on WRITE_SIZE):
 # profiles/k_tran_pack_ring_mc_4096x10ns_traffic.md
-MEASURED_TRAFFIC = {(4096, 10e-9, 2024, "lds"): (1.18950367e8 + 2.628807872e9) * 1024}
+MEASURED_TRAFFIC = {(4096, 10e-9, 2024, "lds"): (2.023831635e8 + 2.5537219226e9) * 1024}
 
 
 def log(msg):
