@@ -293,12 +293,13 @@ __device__ __forceinline__ void x_task(const uint2* xt, const uint32_t* up, int 
     const uint2 t = xt[q];
     if ((t.y >> 16) & 2) return;
     const int k = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
+    const double sc = (t.y >> 16) & 1 ? v.inv[k] : 1.0;
     double acc = v.x[k];
     for (int w = ub; w < ue; w++) {
         const uint32_t ab = up[w];
         acc -= v.F[ab & 0xFFFF] * v.x[ab >> 16];
     }
-    v.x[k] = (t.y >> 16) & 1 ? acc * v.inv[k] : acc;
+    v.x[k] = (t.y >> 16) & 1 ? acc * sc : acc;
 }
 
 // A long X task by the whole wavefront: x[k] minus the products F[a] x[b],
@@ -362,13 +363,16 @@ __device__ __forceinline__ void factor(const CPlan& c, const View& v) {
                 const uint2 t = ft[fb + q];
                 const int e = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
                 const uint32_t ctl = t.y >> 16;
+                // the scale is loaded with the operands (its pivot is from an
+                // earlier level), not after the updates
+                const double sc = ctl < CP_PLAIN_U ? inv[ctl] : 1.0;
                 double acc = F[e];
                 for (int w = ub; w < ue; w++) {
                     const uint32_t ab = up[w];
                     acc -= F[ab & 0xFFFF] * F[ab >> 16];
                 }
                 if (ctl < CP_PLAIN_U) {
-                    F[e] = acc * inv[ctl];
+                    F[e] = acc * sc;
                 } else {
                     F[e] = acc;
                     if (ctl & CP_DIAG) inv[ctl & 0x7FFF] = 1.0 / acc;
