@@ -394,20 +394,23 @@ __global__ __launch_bounds__(kBT) void k_mf_orig(const BigDev* P, const double* 
 }
 
 // LU without pivoting of the nb x nb block D (column-major, leading
-// dimension kNb, in LDS) by the kBT threads of a workgroup, each holding four
-// entries in registers: thread t owns column t / 8, rows t % 8 + 8v.  Before
-// step p the owners of column p and of row p publish them in cb / rb
-// (double-buffered, one barrier per pivot).  The operations and their order
-// are those of the column sweep: l(i,p) = a(i,p) * (1 / u(p,p)), then a(i,j)
-// -= l(i,p) u(p,j) for pivots p ascending.  L (unit diagonal, below) and U
-// overwrite D; ipv[p] = 1 / u(p,p).
+// dimension kNb, in LDS) by NT threads (NT a multiple of 32, at most
+// 1024), each holding 1024 / NT entries in registers: with G = NT / 32 row
+// groups, thread t owns column t / G, rows t % G + G v.  Before step p the
+// owners of column p and of row p publish them in cb / rb (double-buffered,
+// one barrier per pivot; with one wavefront the barrier is only a wavefront
+// one).  The operations and their order are those of the column sweep:
+// l(i,p) = a(i,p) * (1 / u(p,p)), then a(i,j) -= l(i,p) u(p,j) for pivots p
+// ascending.  L (unit diagonal, below) and U overwrite D; ipv[p] = 1 / u(p,p).
+template <int NT>
 __device__ __forceinline__ void diag_lu_block(double* D, double* ipv, int nb, double (*cb)[kNb],
                                               double (*rb)[kNb]) {
-    const int t = threadIdx.x, j = t >> 3, i0 = t & 7;
-    double a[4];
+    constexpr int G = NT / kNb, EP = kNb / G;
+    const int t = threadIdx.x, j = t / G, i0 = t % G;
+    double a[EP];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int i = i0 + 8 * v;
+    for (int v = 0; v < EP; v++) {
+        const int i = i0 + G * v;
         a[v] = (i < nb && j < nb) ? D[j * kNb + i] : 0.0;
         if (j == 0) cb[0][i] = a[v];
         if (i == 0) rb[0][j] = a[v];
@@ -419,8 +422,8 @@ __device__ __forceinline__ void diag_lu_block(double* D, double* ipv, int nb, do
         if (t == 0) ipv[p] = ip;
         const double u = rb[b][j];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = i0 + 8 * v;
+        for (int v = 0; v < EP; v++) {
+            const int i = i0 + G * v;
             if (i > p) {
                 const double l = cb[b][i] * ip;
                 if (j == p) a[v] = l;
@@ -432,14 +435,18 @@ __device__ __forceinline__ void diag_lu_block(double* D, double* ipv, int nb, do
         __syncthreads();
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int i = i0 + 8 * v;
+    for (int v = 0; v < EP; v++) {
+        const int i = i0 + G * v;
         if (i < nb && j < nb) D[j * kNb + i] = a[v];
     }
 }
 
-// item = (front, p0): LU of the diagonal block [p0, p0+nb) x [p0, p0+nb)
-__global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, double* dinv, const int2* items) {
+// item = (front, p0): LU of the diagonal block [p0, p0+nb) x [p0, p0+nb),
+// four wavefronts (four entries per thread; one wavefront was 3x slower: it
+// issues each step's updates serially)
+constexpr int kDiagThreads = kBT;
+__global__ __launch_bounds__(kDiagThreads) void k_mf_diag(const BigDev* P, double* pool, double* dinv,
+                                                          const int2* items) {
     __shared__ double D[kNb * kNb];
     __shared__ double cb[2][kNb], rb[2][kNb];
     const int2 it = items[blockIdx.x];
@@ -447,14 +454,14 @@ __global__ __launch_bounds__(kBT) void k_mf_diag(const BigDev* P, double* pool, 
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int nb = (k - p0) < kNb ? (k - p0) : kNb;
     double* G = pool + P->sn_off[s];
-    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+    for (int q = threadIdx.x; q < nb * nb; q += kDiagThreads) {
         const int j = q / nb, i = q % nb;
         D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
     }
     __syncthreads();
-    diag_lu_block(D, dinv + f0 + p0, nb, cb, rb);
+    diag_lu_block<kDiagThreads>(D, dinv + f0 + p0, nb, cb, rb);
     __syncthreads();
-    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+    for (int q = threadIdx.x; q < nb * nb; q += kDiagThreads) {
         const int j = q / nb, i = q % nb;
         G[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
     }
@@ -750,7 +757,7 @@ __global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* poo
             D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
         }
         __syncthreads();
-        diag_lu_block(D, invd, nb, cb, rb);
+        diag_lu_block<kBT>(D, invd, nb, cb, rb);
         __syncthreads();
         for (int q = threadIdx.x; q < nb * nb; q += kBT) {
             const int j = q / nb, i = q % nb;
@@ -1355,7 +1362,7 @@ void launch_level(BigCircuit& C, hipStream_t s, int l, bool fused) {
         MF_LAUNCH(CS_MFK_ORIG, l, k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval,
                   C.pool, C.d_borig, Lv.orig_b, Lv.orig_e);
     for (auto& pn : Lv.panels) {
-        MF_LAUNCH(CS_MFK_DIAG, l, k_mf_diag, dim3(pn.diag_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
+        MF_LAUNCH(CS_MFK_DIAG, l, k_mf_diag, dim3(pn.diag_n), dim3(kDiagThreads), 0, s, P, C.pool, C.dinv,
                   C.d_diag + pn.diag_b);
         if (pn.trsm_n)
             MF_LAUNCH(CS_MFK_TRSM, l, k_mf_trsm, dim3(pn.trsm_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
