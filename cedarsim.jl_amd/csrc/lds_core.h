@@ -619,18 +619,30 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
                 kout++;
             }
         }
+        // with only DC sources there is no breakpoint and, in transient mode,
+        // no source value that changes with t: the per-step breakpoint
+        // search is skipped and the sources are evaluated at the first step
+        int dyn = 0;
+        for (int s = tid; s < c.n_src; s += W)
+            if (B[c.o_src + 4 * s] != CS_WAVE_DC) dyn = 1;
+        const bool dyn_src = block_reduce<W>((double)dyn, nullptr, OpMax()) > 0.0;
+        bool src_set = false;
         lds_sync();
         while (rc == CS_OK && t < o.t_stop) {
             if (steps + rej >= o.max_steps) { rc = CS_E_MAXSTEPS; break; }
-            double lb = INFINITY;
-            for (int s = tid; s < c.n_src; s += W) {
-                const uint16_t* d = B + c.o_src + 4 * s;
-                const double b = cold_next_bp(d[0], v.wd + d[1], d[2], t);
-                if (b < lb) lb = b;
+            double tb = o.t_stop;
+            bool srcbp = false;
+            if (dyn_src) {
+                double lb = INFINITY;
+                for (int s = tid; s < c.n_src; s += W) {
+                    const uint16_t* d = B + c.o_src + 4 * s;
+                    const double b = cold_next_bp(d[0], v.wd + d[1], d[2], t);
+                    if (b < lb) lb = b;
+                }
+                tb = block_reduce<W>(lb, nullptr, OpMin());
+                srcbp = tb < o.t_stop;
+                if (!srcbp) tb = o.t_stop;
             }
-            double tb = block_reduce<W>(lb, nullptr, OpMin());
-            const bool srcbp = tb < o.t_stop;
-            if (!srcbp) tb = o.t_stop;
             double hh = h < sd.dt_max ? h : sd.dt_max;
             int hit;
             const double tn = clip_step(t, hh, tb, hit);
@@ -651,7 +663,8 @@ __device__ void run_lds(Rec& rec, const CPlan& c, const DPlan* __restrict__ P,
                     qx[r] = a1 * Qh0[r] + (k == 2 ? a2 * Qh1[r] : 0.0);
                 }
             }
-            eval_sources(c, v, tn, CS_MODE_TRAN);
+            if (dyn_src || !src_set) eval_sources(c, v, tn, CS_MODE_TRAN);
+            src_set = true;
             lds_sync();
             if (!newton<NR, NM>(c, v, dr, gmin, a0, qx, pinv, flag, o.newton_maxit, o.newton_tol,
                                 o.reltol, o.abstol, nit)) {
