@@ -819,6 +819,14 @@ __global__ __launch_bounds__(kBT) void k_mf_lvl_big(const BigDev* P, double* poo
 }
 
 // ---- multifrontal solves ----------------------------------------------------------------
+// Lane src's value (src uniform) for the whole wavefront, by readlane.
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // forward: y = L^-1 b by levels; update vectors of the fronts in uvec
 __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
                                                 const int32_t* fronts, int ld_off) {
@@ -852,7 +860,7 @@ __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* p
             const int i = b0 + lane;
             double vi = i < b1 ? v[i] : 0.0;
             for (int p = b0; p < b1; p++) {
-                const double y = __shfl(vi, p - b0, 64);
+                const double y = wave_bcast(vi, p - b0);
                 if (i > p && i < b1) vi -= Ld[(p - b0) * 65 + lane] * y;
             }
             if (i < b1) v[i] = vi;
@@ -909,7 +917,7 @@ __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* p
             double wi = i < b1 ? w[i] : 0.0;
             for (int p = b1 - 1; p >= b0; p--) {
                 if (i == p) wi *= dinv[f0 + p];
-                const double xp = __shfl(wi, p - b0, 64);
+                const double xp = wave_bcast(wi, p - b0);
                 if (i < p) wi -= Ud[(p - b0) * 65 + lane] * xp;
             }
             if (i < b1) w[i] = wi;

@@ -122,3 +122,21 @@ def test_power_grid_full_size(engine):
     sol = s.tran(o, p)
     _say(f"  20 ps transient: {sol.retcode}, {sol.stats.steps} steps, {sol.stats.newton_iters} iterations")
     assert sol.retcode == "Success"
+
+
+def test_full_span_matches_oracle(engine):
+    """The power-grid configuration's whole 0-5 ns span (five pulse periods,
+    breakpoints at every corner) on the supernodal path against the oracle:
+    identical step counts, waveforms within reltol 1e-6 / abstol 1e-9 V."""
+    f = rc_mesh(12, 12, 12, seed=2)
+    s = engine.load(f, solver=L.SOLVER_SUPERNODAL)
+    p = f.params()
+    o = ch.TranOptions(t_start=0.0, t_stop=5e-9, reltol=1e-6, abstol=1e-9, n_out=51, dt_out=1e-10)
+    sol = s.tran(o, p)
+    w, uf, st, rc = O.tran(f, p, None, o)
+    _say(f"12x12 mesh 0-5 ns: {sol.stats.steps} steps (oracle {st.steps}), {sol.stats.newton_iters} iterations")
+    assert sol.retcode == "Success" and rc == 0
+    assert sol.stats.steps == st.steps and sol.stats.rejected == st.rejected
+    wb = np.asarray(sol.wave)
+    wr = np.asarray(w).reshape(wb.shape)
+    assert np.all(np.abs(wb - wr) <= 1e-6 * np.abs(wr) + 1e-9)
