@@ -38,7 +38,7 @@ INIT_DCOP, INIT_UIC = 0, 1
 BATCH_KERNEL_AUTO, BATCH_KERNEL_LDS, BATCH_KERNEL_GLOBAL = 0, 1, 2
 SOLVER_AUTO, SOLVER_LEVEL, SOLVER_SUPERNODAL = 0, 1, 2
 # launch kinds of cs_big_profile (CS_MFK_*)
-MFK_NAMES = ("small32", "small64", "small96", "gather", "orig", "diag", "trsm", "gemm", "front", "fwd", "bwd")
+MFK_NAMES = ("small32", "small64", "small96", "gather", "orig", "diag", "trsm", "gemm", "front", "fwd", "bwd", "panel", "flush")
 SUPERNODAL_MIN_N = 8192
 
 _I = POINTER(c_int32)

@@ -559,22 +559,10 @@ __global__ __launch_bounds__(kBT) void k_mf_trsm(const BigDev* P, double* pool, 
     }
 }
 
-// tile = (front, p0, i0, j0): F[i0:i0+64, j0:j0+64] -= L[.., p0:p0+nb] U[p0:p0+nb, ..]
-__global__ __launch_bounds__(kBT) void k_mf_gemm(const BigDev* P, double* pool, const int4* tiles) {
-    __shared__ double A[kNb][kTile + 1];   // L panel, transposed: A[c][i]
-    __shared__ double Bt[kNb][kTile + 1];  // U panel: Bt[c][j]
-    const int4 t = tiles[blockIdx.x];
-    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
-    const int m = P->sn_m[s], k = sn_k(P, s);
-    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
-    double* G = pool + P->sn_off[s];
-    for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
-        const int c = q / kTile, i = q % kTile;
-        const bool ok = c < nb;
-        A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
-        Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
-    }
-    __syncthreads();
+// F[i0:i0+64, j0:j0+64] -= L U for the L panel rows A[c][i] (transposed) and
+// the U panel columns Bt[c][j] staged in LDS, on the vector ALUs
+__device__ __forceinline__ void tile_update_valu(const double (*A)[kTile + 1], const double (*Bt)[kTile + 1],
+                                                 double* G, int m, int i0, int j0, int nb) {
     const int ti = (threadIdx.x % 16) * 4, tj = (threadIdx.x / 16) * 4;
     double acc[4][4] = {};
     for (int c = 0; c < nb; c++) {
@@ -593,26 +581,38 @@ __global__ __launch_bounds__(kBT) void k_mf_gemm(const BigDev* P, double* pool, 
     }
 }
 
-// The same update on the FP64 matrix cores: four wavefronts, each a 32 x 32
-// quarter of the tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A: lane l
-// holds A[l%16][l/16], B: B[l/16][l%16], accumulator element v of lane l is
-// D[l/16 + 4v][l%16], as measured on gfx950).
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* pool, const int4* tiles) {
-    __shared__ double A[kNb][kTile + 1];
-    __shared__ double Bt[kNb][kTile + 1];
-    const int4 t = tiles[blockIdx.x];
-    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
-    const int m = P->sn_m[s], k = sn_k(P, s);
-    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
-    double* G = pool + P->sn_off[s];
+// The panel rows i0.. of L and columns j0.. of U into LDS (zero past m and nb)
+__device__ __forceinline__ void load_panel_tiles(double (*A)[kTile + 1], double (*Bt)[kTile + 1], const double* G,
+                                                 int m, int p0, int i0, int j0, int nb) {
     for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
         const int c = q / kTile, i = q % kTile;
         const bool ok = c < nb;
         A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
         Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
     }
+}
+
+// tile = (front, p0, i0, j0): F[i0:i0+64, j0:j0+64] -= L[.., p0:p0+nb] U[p0:p0+nb, ..]
+__global__ __launch_bounds__(kBT) void k_mf_gemm(const BigDev* P, double* pool, const int4* tiles) {
+    __shared__ double A[kNb][kTile + 1];   // L panel, transposed: A[c][i]
+    __shared__ double Bt[kNb][kTile + 1];  // U panel: Bt[c][j]
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
+    const int m = P->sn_m[s], k = sn_k(P, s);
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    load_panel_tiles(A, Bt, G, m, p0, i0, j0, nb);
     __syncthreads();
+    tile_update_valu(A, Bt, G, m, i0, j0, nb);
+}
+
+// The same update on the FP64 matrix cores: four wavefronts, each a 32 x 32
+// quarter of the tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A: lane l
+// holds A[l%16][l/16], B: B[l/16][l%16], accumulator element v of lane l is
+// D[l/16 + 4v][l%16], as measured on gfx950).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void tile_update_mfma(const double (*A)[kTile + 1], const double (*Bt)[kTile + 1],
+                                                 double* G, int m, int i0, int j0, int nb) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
     const int r16 = lane & 15, q4 = lane >> 4;
@@ -636,6 +636,116 @@ __global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* p
                 if (i < m) G[(int64_t)j * m + i] -= acc[bi][bj][v];
             }
         }
+}
+
+__global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* pool, const int4* tiles) {
+    __shared__ double A[kNb][kTile + 1];
+    __shared__ double Bt[kNb][kTile + 1];
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, p0 = t.y, i0 = t.z, j0 = t.w;
+    const int m = P->sn_m[s], k = sn_k(P, s);
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb;
+    double* G = pool + P->sn_off[s];
+    load_panel_tiles(A, Bt, G, m, p0, i0, j0, nb);
+    __syncthreads();
+    tile_update_mfma(A, Bt, G, m, i0, j0, nb);
+}
+
+// One step of the blocked front LU in one launch.  item = (front, p0, i0,
+// j0): every workgroup factors the diagonal block [p0, p1) itself, solves
+// the L rows [i0, i0+64) and the U columns [j0, j0+64) it needs, and
+// updates the trailing tile F[i0:, j0:]; (front, p0, -1, -1) is a step with
+// no trailing matrix (p1 = m).  The operations and their order are those of
+// k_mf_diag, k_mf_trsm and the trailing update, so the factors are the same
+// bits.  Workgroups of the step read the unfactored panel from the front,
+// so the factored panel goes to the workspace W instead (the same layout,
+// at woff[front]): the diagonal block from the tile (p1, p1) or the item
+// without trailing matrix, L rows from the tiles with j0 = p1, U columns
+// from those with i0 = p1.  k_mf_flush copies it back once the level's
+// steps are done; later steps read only the trailing matrix.
+template <bool Mfma>
+__global__ __launch_bounds__(kBT) void k_mf_panel(const BigDev* P, double* pool, double* dinv, double* W,
+                                                  const int64_t* woff, const int4* items) {
+    __shared__ double D[kNb * kNb];
+    __shared__ double cb[2][kNb], rb[2][kNb];
+    __shared__ double ipv[kNb];
+    __shared__ double EU[kCoef], EL[kCoef];
+    __shared__ double A[kNb][kTile + 1];
+    __shared__ double Bt[kNb][kTile + 1];
+    const int4 it = items[blockIdx.x];
+    const int s = it.x, p0 = it.y, i0 = it.z, j0 = it.w;
+    const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
+    const int nb = (k - p0) < kNb ? (k - p0) : kNb, p1 = p0 + nb;
+    double* G = pool + P->sn_off[s];
+    double* Wf = W + woff[s];
+    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+        const int j = q / nb, i = q % nb;
+        D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
+    }
+    if (i0 >= 0) load_panel_tiles(A, Bt, G, m, p0, i0, j0, nb);
+    __syncthreads();
+    diag_lu_block<kBT>(D, ipv, nb, cb, rb);
+    __syncthreads();
+    const bool diag_owner = i0 < 0 || (i0 == p1 && j0 == p1);
+    if (diag_owner) {
+        for (int q = threadIdx.x; q < nb * nb; q += kBT) {
+            const int j = q / nb, i = q % nb;
+            Wf[(int64_t)(p0 + j) * m + p0 + i] = D[j * kNb + i];
+        }
+        if ((int)threadIdx.x < nb) dinv[f0 + p0 + threadIdx.x] = ipv[threadIdx.x];
+    }
+    if (i0 < 0) return;
+    load_coef<true>(EU, D, kNb, nb);
+    load_coef<false>(EL, D, kNb, nb);
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < kTile) {
+        if (i0 + t < m) {
+            double a[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; c++) a[c] = c < nb ? A[c][t] : 0.0;
+            rot_solve<true>(a, EU, ipv, nb, [&](int c, double x) { A[c][t] = x; });
+        }
+    } else if (t < 2 * kTile) {
+        const int jj = t - kTile;
+        if (j0 + jj < m) {
+            double a[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; c++) a[c] = c < nb ? Bt[c][jj] : 0.0;
+            rot_solve<false>(a, EL, nullptr, nb, [&](int c, double x) { Bt[c][jj] = x; });
+        }
+    }
+    __syncthreads();
+    if (j0 == p1)
+        for (int q = threadIdx.x; q < nb * kTile; q += kBT) {
+            const int c = q / kTile, i = q % kTile;
+            if (i0 + i < m) Wf[(int64_t)(p0 + c) * m + i0 + i] = A[c][i];
+        }
+    if (i0 == p1)
+        for (int q = threadIdx.x; q < nb * kTile; q += kBT) {
+            const int jj = q / nb, c = q % nb;
+            if (j0 + jj < m) Wf[(int64_t)(j0 + jj) * m + p0 + c] = Bt[c][jj];
+        }
+    if (Mfma) tile_update_mfma(A, Bt, G, m, i0, j0, nb);
+    else tile_update_valu(A, Bt, G, m, i0, j0, nb);
+}
+
+// item = (front, j0): copies the factored panels of columns [j0, j0+4) back
+// from the workspace, i.e. the entries (i, j) with min(i, j) < k: the whole
+// column for j < k, its rows [0, k) for j >= k
+constexpr int kFlushCols = 4;
+__global__ __launch_bounds__(kBT) void k_mf_flush(const BigDev* P, double* pool, const double* W,
+                                                  const int64_t* woff, const int2* items) {
+    const int2 it = items[blockIdx.x];
+    const int s = it.x, m = P->sn_m[s], k = sn_k(P, s);
+    double* G = pool + P->sn_off[s];
+    const double* Wf = W + woff[s];
+    const int j1 = (it.y + kFlushCols) < m ? it.y + kFlushCols : m;
+    for (int j = it.y; j < j1; j++) {
+        const int rows = j < k ? m : k;
+        const int64_t o = (int64_t)j * m;
+        for (int i = threadIdx.x; i < rows; i += kBT) G[o + i] = Wf[o + i];
+    }
 }
 
 // ---- fused per-level front kernels (one workgroup per front) ------------------------------
@@ -1058,12 +1168,14 @@ struct BigCircuit {
     // launch tables
     struct Panel {
         int diag_b, diag_n, trsm_b, trsm_n, gemm_b, gemm_n;
+        int pan_b, pan_n;   // one-launch form (k_mf_panel)
     };
     struct Level {
         int tile_b, tile_n, fr_b, fr_n, orig_b, orig_e, big_b, big_n;
         int small_b[kSmallBuckets], small_n[kSmallBuckets];   // fronts up to kBucketMax[bucket]
         size_t small_lds[kSmallBuckets], solve_lds;
         int ld_off;   // offset of the solves' diagonal block in their LDS
+        int flush_b, flush_n;
         std::vector<Panel> panels;
     };
     std::vector<Level> levels;
@@ -1073,6 +1185,10 @@ struct BigCircuit {
     int2* d_diag = nullptr;
     int4* d_trsm = nullptr;
     int4* d_gemm = nullptr;
+    int4* d_pan = nullptr;
+    int2* d_flush = nullptr;
+    int64_t* d_woff = nullptr;   // per front: offset of its factored panels in wpan
+    double* wpan = nullptr;
     int32_t* d_borig = nullptr;   // Jacobian positions of the large fronts, by level
     void* tbuf = nullptr;
     hipGraph_t graph = nullptr;
@@ -1080,6 +1196,9 @@ struct BigCircuit {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double factor_ms = 0.0, solve_ms = 0.0;
     bool mfma = false;   // trailing updates on the matrix cores
+    // panel steps as diagonal block, solves and trailing update in three
+    // launches instead of k_mf_panel's one (CEDAR_MF_PANEL=split)
+    bool split_panels = false;
     bool use_graph = true;
     // per level: fused (one workgroup per front) or multi-launch panels;
     // chosen by tune_levels unless CEDAR_MF_FRONT=fused|panel fixes it
@@ -1199,9 +1318,11 @@ int upload(BigCircuit& C, std::string& err) {
     BCHK(hipMemcpy(C.d_dev, &C.hd, sizeof(BigDev), hipMemcpyHostToDevice));
 
     // launch tables
-    std::vector<int4> tiles, trsm, gemm;
+    std::vector<int4> tiles, trsm, gemm, pan;
     std::vector<int32_t> fronts, borig;
-    std::vector<int2> diag, cbr;
+    std::vector<int2> diag, cbr, flush;
+    std::vector<int64_t> woff(std::max(B.ns, 1), 0);
+    int64_t wpan_n = 0;   // largest sum of m^2 over one level's large fronts
     // the scatter gathers find a tile's block columns by bisection of the
     // relative maps, which the symbolic analysis builds ascending
     for (int c = 0; c < B.ns; c++)
@@ -1266,12 +1387,23 @@ int upload(BigCircuit& C, std::string& err) {
         Lv.ld_off = (std::max(m_max, 1) + 1) & ~1;
         Lv.solve_lds = sizeof(double) * ((size_t)Lv.ld_off + (size_t)std::min(k_lev, 64) * 65);
         int kmax = 0;
-        for (int s : large) kmax = std::max(kmax, B.sn_first[s + 1] - B.sn_first[s]);
+        int64_t wl = 0;
+        Lv.flush_b = (int)flush.size();
+        for (int s : large) {
+            const int m = B.sn_m[s];
+            kmax = std::max(kmax, B.sn_first[s + 1] - B.sn_first[s]);
+            woff[s] = wl;
+            wl += ((int64_t)m * m + 1) & ~int64_t(1);
+            for (int j0 = 0; j0 < m; j0 += kFlushCols) flush.push_back(make_int2(s, j0));
+        }
+        Lv.flush_n = (int)flush.size() - Lv.flush_b;
+        wpan_n = std::max(wpan_n, wl);
         for (int p0 = 0; p0 < kmax; p0 += kNb) {
             BigCircuit::Panel pn{};
             pn.diag_b = (int)diag.size();
             pn.trsm_b = (int)trsm.size();
             pn.gemm_b = (int)gemm.size();
+            pn.pan_b = (int)pan.size();
             for (int s : large) {
                 const int k = B.sn_first[s + 1] - B.sn_first[s], m = B.sn_m[s];
                 if (p0 >= k) continue;
@@ -1280,8 +1412,13 @@ int upload(BigCircuit& C, std::string& err) {
                 for (int r0 = p1; r0 < m; r0 += kBT) trsm.push_back(make_int4(s, p0, 0, r0));
                 for (int r0 = p1; r0 < m; r0 += kTrsmCols) trsm.push_back(make_int4(s, p0, 1, r0));
                 for (int j0 = p1; j0 < m; j0 += kTile)
-                    for (int i0 = p1; i0 < m; i0 += kTile) gemm.push_back(make_int4(s, p0, i0, j0));
+                    for (int i0 = p1; i0 < m; i0 += kTile) {
+                        gemm.push_back(make_int4(s, p0, i0, j0));
+                        pan.push_back(make_int4(s, p0, i0, j0));
+                    }
+                if (p1 >= m) pan.push_back(make_int4(s, p0, -1, -1));
             }
+            pn.pan_n = (int)pan.size() - pn.pan_b;
             pn.diag_n = (int)diag.size() - pn.diag_b;
             pn.trsm_n = (int)trsm.size() - pn.trsm_b;
             pn.gemm_n = (int)gemm.size() - pn.gemm_b;
@@ -1290,7 +1427,8 @@ int upload(BigCircuit& C, std::string& err) {
     }
     std::vector<char> tb;
     size_t o_t = pack_add(tb, tiles), o_f = pack_add(tb, fronts), o_d = pack_add(tb, diag),
-           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm), o_bo = pack_add(tb, borig), o_cb = pack_add(tb, cbr);
+           o_tr = pack_add(tb, trsm), o_g = pack_add(tb, gemm), o_bo = pack_add(tb, borig), o_cb = pack_add(tb, cbr),
+           o_pa = pack_add(tb, pan), o_fl = pack_add(tb, flush), o_wo = pack_add(tb, woff);
     BCHK(hipMalloc(&C.tbuf, tb.size()));
     BCHK(hipMemcpy(C.tbuf, tb.data(), tb.size(), hipMemcpyHostToDevice));
     char* t = (char*)C.tbuf;
@@ -1301,12 +1439,15 @@ int upload(BigCircuit& C, std::string& err) {
     C.d_gemm = (int4*)(t + o_g);
     C.d_borig = (int32_t*)(t + o_bo);
     C.d_cbr = (int2*)(t + o_cb);
+    C.d_pan = (int4*)(t + o_pa);
+    C.d_flush = (int2*)(t + o_fl);
+    C.d_woff = (int64_t*)(t + o_wo);
 
     // workspace
     const size_t n = B.n, nnz = P.nnz_nat;
     const size_t nd = P.lay.n_params + P.n_lin + 3 * nnz + (size_t)B.pool + 16 * n + P.n_stamp +
                       B.rel_ptr[B.ns] + 2 * kRowBlocks + (size_t)C.hd.n_long + (size_t)C.hd.n_chunks + 32 +
-                      NRM_SLOTS + 8 +
+                      NRM_SLOTS + 8 + (size_t)wpan_n +
                       sizeof(StepDev) / 8 + 64;
     BCHK(hipMalloc(&C.wbuf, nd * sizeof(double)));
     BCHK(hipMemset(C.wbuf, 0, nd * sizeof(double)));
@@ -1318,6 +1459,7 @@ int upload(BigCircuit& C, std::string& err) {
     C.Clin = take(nnz);
     C.Aval = take(nnz);
     C.pool = take(B.pool);
+    C.wpan = take((size_t)wpan_n);
     C.dinv = take(n);
     C.u = take(n);
     C.x = take(n);
@@ -1369,6 +1511,19 @@ void launch_level(BigCircuit& C, hipStream_t s, int l, bool fused) {
     if (Lv.orig_e > Lv.orig_b)
         MF_LAUNCH(CS_MFK_ORIG, l, k_mf_orig, dim3(grid_for(Lv.orig_e - Lv.orig_b)), dim3(kBT), 0, s, P, C.Aval,
                   C.pool, C.d_borig, Lv.orig_b, Lv.orig_e);
+    if (!C.split_panels) {
+        for (auto& pn : Lv.panels) {
+            if (C.mfma)
+                MF_LAUNCH(CS_MFK_PANEL, l, k_mf_panel<true>, dim3(pn.pan_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
+                          C.wpan, C.d_woff, C.d_pan + pn.pan_b);
+            else
+                MF_LAUNCH(CS_MFK_PANEL, l, k_mf_panel<false>, dim3(pn.pan_n), dim3(kBT), 0, s, P, C.pool, C.dinv,
+                          C.wpan, C.d_woff, C.d_pan + pn.pan_b);
+        }
+        MF_LAUNCH(CS_MFK_FLUSH, l, k_mf_flush, dim3(Lv.flush_n), dim3(kBT), 0, s, P, C.pool, C.wpan, C.d_woff,
+                  C.d_flush + Lv.flush_b);
+        return;
+    }
     for (auto& pn : Lv.panels) {
         MF_LAUNCH(CS_MFK_DIAG, l, k_mf_diag, dim3(pn.diag_n), dim3(kDiagThreads), 0, s, P, C.pool, C.dinv,
                   C.d_diag + pn.diag_b);
@@ -1563,6 +1718,8 @@ int big_create(const cs_netlist* nl, hipStream_t s, BigCircuit** out, std::strin
     // CEDAR_MF_GEMM=valu asks for the vector-ALU kernel (A/B comparison)
     const char* g = std::getenv("CEDAR_MF_GEMM");
     C->mfma = !(g && std::strcmp(g, "valu") == 0);
+    const char* sp = std::getenv("CEDAR_MF_PANEL");
+    C->split_panels = sp && std::strcmp(sp, "split") == 0;
     const char* fm = std::getenv("CEDAR_MF_FRONT");
     const bool fix_fused = fm && std::strcmp(fm, "fused") == 0;
     const bool fix_panel = fm && std::strcmp(fm, "panel") == 0;

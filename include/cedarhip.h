@@ -238,6 +238,8 @@ int cs_big_times(cs_ctx* ctx, int32_t cid, double* factor_ms, double* solve_ms);
 #define CS_MFK_FRONT   8   /* one workgroup per large front (the fused form)          */
 #define CS_MFK_FWD     9   /* forward substitution of one level                        */
 #define CS_MFK_BWD     10  /* backward substitution of one level                       */
+#define CS_MFK_PANEL   11  /* one panel step in one launch (diagonal block, solves, update) */
+#define CS_MFK_FLUSH   12  /* factored panels of a level back from the workspace          */
 int cs_big_profile(cs_ctx* ctx, int32_t cid, int32_t* kinds, int32_t* levels, float* ms, int32_t cap,
                    int32_t* count);
 

@@ -51,6 +51,39 @@ def test_factor_solve_matches_sparse_lu(engine, nx, ninv):
         assert err < 1e-9
 
 
+@pytest.mark.parametrize("gemm", ["mfma", "valu"])
+def test_panel_step_forms_agree_bitwise(engine, monkeypatch, gemm):
+    """A panel step of the large fronts as one launch (each workgroup factors
+    the diagonal block and solves its own panel rows and columns) and as
+    three launches (diagonal block, solves, trailing update) perform the same
+    operations in the same order: the factors and solutions are the same
+    bits.  The panel form is forced on every level."""
+    f = rc_mesh(110, 110, 400, seed=5)
+    monkeypatch.setenv("CEDAR_MF_FRONT", "panel")
+    monkeypatch.setenv("CEDAR_MF_GEMM", gemm)
+    rng = np.random.default_rng(11)
+    p = f.params()
+    xs = {}
+    for form in ("one", "split"):
+        if form == "split":
+            monkeypatch.setenv("CEDAR_MF_PANEL", "split")
+        else:
+            monkeypatch.delenv("CEDAR_MF_PANEL", raising=False)
+        s = engine.load(f, solver=L.SOLVER_SUPERNODAL)
+        if form == "one":
+            u = rng.uniform(0, 1, s.n)
+            b = rng.standard_normal(s.n)
+        A = s.jac_matrix(None, u, p, 1e13, 0.3e-9)
+        s.lu_refactor(A.data)
+        xs[form] = s.lu_solve(b)
+        _say(f"panel form {form} ({gemm}): max front {s.big_info.max_front}, "
+             f"max |x| {np.max(np.abs(xs[form])):.3e}")
+    assert np.array_equal(xs["one"], xs["split"])
+    berr = np.max(np.abs(A @ xs["one"] - b)) / (abs(A).sum(axis=1).max() * np.max(np.abs(xs["one"])) +
+                                                np.max(np.abs(b)))
+    assert berr < 1e-14
+
+
 def test_jacobian_residual_match_level_path(engine):
     f = rc_mesh(20, 20, 40, seed=5)
     s = engine.load(f, solver=L.SOLVER_SUPERNODAL)
