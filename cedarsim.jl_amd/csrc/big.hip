@@ -581,15 +581,44 @@ __device__ __forceinline__ void tile_update_valu(const double (*A)[kTile + 1], c
     }
 }
 
-// The panel rows i0.. of L and columns j0.. of U into LDS (zero past m and nb)
+// The panel rows i0.. of L and columns j0.. of U into LDS (zero past m and
+// nb).  All loads are issued before the first LDS store, so the 16 global
+// round trips overlap; consecutive threads read consecutive addresses (L:
+// down a column of the panel, U: along a panel row's column segment).
 __device__ __forceinline__ void load_panel_tiles(double (*A)[kTile + 1], double (*Bt)[kTile + 1], const double* G,
                                                  int m, int p0, int i0, int j0, int nb) {
-    for (int q = threadIdx.x; q < kNb * kTile; q += kBT) {
-        const int c = q / kTile, i = q % kTile;
-        const bool ok = c < nb;
-        A[c][i] = (ok && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
-        Bt[c][i] = (ok && j0 + i < m) ? G[(int64_t)(j0 + i) * m + p0 + c] : 0.0;
+    constexpr int R = kNb * kTile / kBT;
+    static_assert(R * kBT == kNb * kTile, "panel tiles split evenly over the threads");
+    double va[R], vb[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = (int)threadIdx.x + r * kBT;
+        const int c = q / kTile, i = q % kTile;       // A: column c, row i
+        const int cu = q % kNb, ju = q / kNb;         // Bt: row cu, column ju
+        va[r] = (c < nb && i0 + i < m) ? G[(int64_t)(p0 + c) * m + i0 + i] : 0.0;
+        vb[r] = (cu < nb && j0 + ju < m) ? G[(int64_t)(j0 + ju) * m + p0 + cu] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = (int)threadIdx.x + r * kBT;
+        A[q / kTile][q % kTile] = va[r];
+        Bt[q % kNb][q / kNb] = vb[r];
+    }
+}
+
+// The nb x nb diagonal block at (p0, p0) of G (leading dimension m) into D
+// (leading dimension kNb), all loads before the first store
+__device__ __forceinline__ void load_diag_block(double* D, const double* G, int m, int p0, int nb) {
+    constexpr int R = kNb * kNb / kBT;
+    static_assert(R * kBT == kNb * kNb, "diagonal block splits evenly over the threads");
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = (int)threadIdx.x + r * kBT, j = q / kNb, i = q % kNb;
+        v[r] = (i < nb && j < nb) ? G[(int64_t)(p0 + j) * m + p0 + i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) D[(int)threadIdx.x + r * kBT] = v[r];
 }
 
 // tile = (front, p0, i0, j0): F[i0:i0+64, j0:j0+64] -= L[.., p0:p0+nb] U[p0:p0+nb, ..]
@@ -638,6 +667,51 @@ __device__ __forceinline__ void tile_update_mfma(const double (*A)[kTile + 1], c
         }
 }
 
+// The tile entries tile_update_mfma's thread subtracts from, read ahead:
+// cv[(2 bi + bj) 4 + v] (zero outside the front)
+__device__ __forceinline__ void tile_load_mfma(const double* G, int m, int i0, int j0, double (&cv)[16]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int r16 = lane & 15, q4 = lane >> 4;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wj + bj * 16 + r16, i = i0 + wi + bi * 16 + q4 + 4 * v;
+                cv[(2 * bi + bj) * 4 + v] = (i < m && j < m) ? G[(int64_t)j * m + i] : 0.0;
+            }
+}
+
+// tile_update_mfma with the tile's entries already in cv
+__device__ __forceinline__ void tile_update_mfma_c(const double (*A)[kTile + 1], const double (*Bt)[kTile + 1],
+                                                   double* G, int m, int i0, int j0, int nb, const double (&cv)[16]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    dbl4 acc[2][2];
+    for (int bi = 0; bi < 2; bi++)
+        for (int bj = 0; bj < 2; bj++) acc[bi][bj] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < nb; kk += 4) {
+        const double a0 = A[kk + q4][wi + r16], a1 = A[kk + q4][wi + 16 + r16];
+        const double b0 = Bt[kk + q4][wj + r16], b1 = Bt[kk + q4][wj + 16 + r16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wj + bj * 16 + r16, i = i0 + wi + bi * 16 + q4 + 4 * v;
+                if (i < m && j < m) G[(int64_t)j * m + i] = cv[(2 * bi + bj) * 4 + v] - acc[bi][bj][v];
+            }
+}
+
 __global__ __launch_bounds__(kBT) void k_mf_gemm_mfma(const BigDev* P, double* pool, const int4* tiles) {
     __shared__ double A[kNb][kTile + 1];
     __shared__ double Bt[kNb][kTile + 1];
@@ -678,11 +752,12 @@ __global__ __launch_bounds__(kBT) void k_mf_panel(const BigDev* P, double* pool,
     const int nb = (k - p0) < kNb ? (k - p0) : kNb, p1 = p0 + nb;
     double* G = pool + P->sn_off[s];
     double* Wf = W + woff[s];
-    for (int q = threadIdx.x; q < nb * nb; q += kBT) {
-        const int j = q / nb, i = q % nb;
-        D[j * kNb + i] = G[(int64_t)(p0 + j) * m + p0 + i];
+    load_diag_block(D, G, m, p0, nb);
+    double cv[16];   // the trailing tile, read ahead of the LU (MFMA form)
+    if (i0 >= 0) {
+        load_panel_tiles(A, Bt, G, m, p0, i0, j0, nb);
+        if (Mfma) tile_load_mfma(G, m, i0, j0, cv);
     }
-    if (i0 >= 0) load_panel_tiles(A, Bt, G, m, p0, i0, j0, nb);
     __syncthreads();
     diag_lu_block<kBT>(D, ipv, nb, cb, rb);
     __syncthreads();
@@ -726,7 +801,7 @@ __global__ __launch_bounds__(kBT) void k_mf_panel(const BigDev* P, double* pool,
             const int jj = q / nb, c = q % nb;
             if (j0 + jj < m) Wf[(int64_t)(j0 + jj) * m + p0 + c] = Bt[c][jj];
         }
-    if (Mfma) tile_update_mfma(A, Bt, G, m, i0, j0, nb);
+    if (Mfma) tile_update_mfma_c(A, Bt, G, m, i0, j0, nb, cv);
     else tile_update_valu(A, Bt, G, m, i0, j0, nb);
 }
 
