@@ -345,7 +345,6 @@ __device__ __forceinline__ void scatter_cb(const BigDev* P, const double* pool, 
     for (int q = threadIdx.x; q < (j1 - j0) * m; q += NT) F[q] = 0.0;
     if (Global) gsync();
     else __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c0 = P->ch_ptr[s], c1 = P->ch_ptr[s + 1];
     for (int cq = c0; cq < c1; cq++) {
         const int c = P->ch_sn[cq];
@@ -358,10 +357,26 @@ __device__ __forceinline__ void scatter_cb(const BigDev* P, const double* pool, 
             b0 = r.x;
             b1 = r.y;
         }
-        for (int b = b0 + wave; b < b1; b += NT / 64) {
-            double* Fc = F + (int64_t)(rl[b] - j0) * m;
-            const double* src = blk + (int64_t)b * mc;
-            for (int a = lane; a < rc; a += 64) Fc[rl[a]] += src[a];
+        // entry (a, b) of the child's block, flattened; the loads of four
+        // entries are issued before their adds.  A child's entries land on
+        // distinct entries of F, so the order within a child is free.
+        constexpr int U = 4;
+        const int tot = (b1 - b0) * rc;
+        for (int q0 = threadIdx.x; q0 < tot; q0 += NT * U) {
+            int dst[U];
+            double sv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int q = q0 + u * NT;
+                if (q < tot) {
+                    const int b = b0 + q / rc, a = q % rc;
+                    dst[u] = (rl[b] - j0) * m + rl[a];
+                    sv[u] = blk[(int64_t)b * mc + a];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (q0 + u * NT < tot) F[dst[u]] += sv[u];
         }
         if (Global) gsync();
         else __syncthreads();
@@ -1013,6 +1028,23 @@ __device__ __forceinline__ double wave_bcast(double v, int src) {
 }
 
 // forward: y = L^-1 b by levels; update vectors of the fronts in uvec
+// The solves' diagonal block: S[pp * 65 + ii] = G[(b0 + pp) m + b0 + ii]
+// for pp < nbk (zero for ii >= nbk), all loads issued before the first store
+__device__ __forceinline__ void stage_solve_block(double* S, const double* G, int m, int b0, int nbk) {
+    constexpr int R = 64 * 64 / kBT;
+    double t[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = (int)threadIdx.x + r * kBT, pp = q >> 6, ii = q & 63;
+        t[r] = (pp < nbk && ii < nbk) ? G[(int64_t)(b0 + pp) * m + b0 + ii] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = (int)threadIdx.x + r * kBT, pp = q >> 6, ii = q & 63;
+        if (pp < nbk) S[pp * 65 + ii] = t[r];
+    }
+}
+
 __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* pool, double* x, double* uvec,
                                                 const int32_t* fronts, int ld_off) {
     extern __shared__ double v[];
@@ -1036,10 +1068,7 @@ __global__ __launch_bounds__(kBT) void k_mf_fwd(const BigDev* P, const double* p
     for (int b0 = 0; b0 < k; b0 += 64) {
         const int b1 = (k - b0) < 64 ? k : b0 + 64;
         const int nbk = b1 - b0;
-        for (int q = threadIdx.x; q < nbk * 64; q += kBT) {
-            const int pp = q >> 6, ii = q & 63;
-            if (pp < nbk) Ld[pp * 65 + ii] = ii < nbk ? G[(int64_t)(b0 + pp) * m + b0 + ii] : 0.0;
-        }
+        stage_solve_block(Ld, G, m, b0, nbk);
         __syncthreads();
         if (threadIdx.x < 64) {
             const int i = b0 + lane;
@@ -1092,10 +1121,7 @@ __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* p
     for (int b1 = k; b1 > 0; b1 -= 64) {
         const int b0 = b1 > 64 ? b1 - 64 : 0;
         const int nbk = b1 - b0;
-        for (int q = threadIdx.x; q < nbk * 64; q += kBT) {
-            const int pp = q >> 6, ii = q & 63;
-            Ud[pp * 65 + ii] = ii < nbk ? G[(int64_t)(b0 + pp) * m + b0 + ii] : 0.0;
-        }
+        stage_solve_block(Ud, G, m, b0, nbk);
         __syncthreads();
         if (threadIdx.x < 64) {
             const int i = b0 + lane;
