@@ -859,9 +859,27 @@ __global__ __launch_bounds__(TL * TL) void k_mf_lvl_small(const BigDev* P, doubl
     const int m = P->sn_m[s], k = sn_k(P, s), f0 = P->sn_first[s];
     const int64_t off = P->sn_off[s];
     scatter_cb<NT, false>(P, pool, F, m, s, 0, m, nullptr);
-    for (int q = P->osn_ptr[s] + threadIdx.x; q < P->osn_ptr[s + 1]; q += NT) {
-        const int e = P->osn_pos[q];
-        F[P->nat_dest[e] - off] += Aval[e];
+    // the front's Jacobian entries (distinct positions: order free), four
+    // per thread with their loads ahead of the adds
+    {
+        constexpr int U = 4;
+        const int q1 = P->osn_ptr[s + 1];
+        for (int q0 = P->osn_ptr[s] + threadIdx.x; q0 < q1; q0 += NT * U) {
+            int e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = q0 + u * NT < q1 ? P->osn_pos[q0 + u * NT] : 0;
+            int64_t d[U];
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (q0 + u * NT < q1) {
+                    d[u] = P->nat_dest[e[u]] - off;
+                    v[u] = Aval[e[u]];
+                }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (q0 + u * NT < q1) F[d[u]] += v[u];
+        }
     }
     __syncthreads();
     const int ti = threadIdx.x % TL, tj = threadIdx.x / TL;
@@ -1126,8 +1144,9 @@ __global__ __launch_bounds__(kBT) void k_mf_bwd(const BigDev* P, const double* p
         if (threadIdx.x < 64) {
             const int i = b0 + lane;
             double wi = i < b1 ? w[i] : 0.0;
+            const double di = i < b1 ? dinv[f0 + i] : 0.0;   // the lane's own 1 / u(i, i)
             for (int p = b1 - 1; p >= b0; p--) {
-                if (i == p) wi *= dinv[f0 + p];
+                if (i == p) wi *= di;
                 const double xp = wave_bcast(wi, p - b0);
                 if (i < p) wi -= Ud[(p - b0) * 65 + lane] * xp;
             }
