@@ -433,17 +433,22 @@ __device__ __forceinline__ void diag_lu_block(double* D, double* ipv, int nb, do
     __syncthreads();
     for (int p = 0; p < nb; p++) {
         const int b = p & 1;
-        const double ip = 1.0 / rb[b][p];
-        if (t == 0) ipv[p] = ip;
+        // every LDS read of the step is issued before the first use, and the
+        // update is computed for all entries and selected, so no entry waits
+        // in a branch of its own
+        const double piv = rb[b][p];
         const double u = rb[b][j];
+        double c[EP];
+#pragma unroll
+        for (int v = 0; v < EP; v++) c[v] = cb[b][i0 + G * v];
+        const double ip = 1.0 / piv;
+        if (t == 0) ipv[p] = ip;
 #pragma unroll
         for (int v = 0; v < EP; v++) {
             const int i = i0 + G * v;
-            if (i > p) {
-                const double l = cb[b][i] * ip;
-                if (j == p) a[v] = l;
-                else if (j > p) a[v] = a[v] - l * u;
-            }
+            const double l = c[v] * ip;
+            const double upd = a[v] - l * u;
+            a[v] = i > p ? (j == p ? l : (j > p ? upd : a[v])) : a[v];
             if (j == p + 1) cb[b ^ 1][i] = a[v];
             if (i == p + 1) rb[b ^ 1][j] = a[v];
         }
@@ -899,15 +904,19 @@ __global__ __launch_bounds__(TL * TL) void k_mf_lvl_small(const BigDev* P, doubl
         const double ip = 1.0 / rb[b][p];
         if (threadIdx.x == 0) dinv[f0 + p] = ip;
         double l[R], u[R];
+        // the pivot row and column are read unconditionally (in bounds: the
+        // buffers hold MX entries) and the update is selected, not branched
 #pragma unroll
         for (int y = 0; y < R; y++) {
             const int i = ti + TL * y;
-            l[y] = (i > p && i < m) ? cb[b][i] * ip : 0.0;
+            const double cv = cb[b][i];
+            l[y] = (i > p && i < m) ? cv * ip : 0.0;
         }
 #pragma unroll
         for (int x = 0; x < R; x++) {
             const int j = tj + TL * x;
-            u[x] = (j > p && j < m) ? rb[b][j] : 0.0;
+            const double rv = rb[b][j];
+            u[x] = (j > p && j < m) ? rv : 0.0;
         }
 #pragma unroll
         for (int y = 0; y < R; y++) {
@@ -917,10 +926,8 @@ __global__ __launch_bounds__(TL * TL) void k_mf_lvl_small(const BigDev* P, doubl
             for (int x = 0; x < R; x++) {
                 if (TL * x >= m) break;
                 const int j = tj + TL * x;
-                if (i > p) {
-                    if (j == p) a[y][x] = l[y];
-                    else if (j > p) a[y][x] = a[y][x] - l[y] * u[x];
-                }
+                const double upd = a[y][x] - l[y] * u[x];
+                a[y][x] = i > p ? (j == p ? l[y] : (j > p ? upd : a[y][x])) : a[y][x];
                 if (j == p + 1) cb[b ^ 1][i] = a[y][x];
                 if (i == p + 1) rb[b ^ 1][j] = a[y][x];
             }
