@@ -289,16 +289,37 @@ __device__ __forceinline__ double assemble(const CPlan& c, const View& v, double
 
 // One X task: x[k] -= sum F[a] x[b]; scaled by inv[k] when fl & 1.  Long
 // tasks (fl & 2) are skipped here and run by x_task_long.
+// acc minus the products Fa[a] Fb[b] over the packed pairs up[w..we) (a in
+// the low 16 bits, b in the high), subtracted in order; the loads of four
+// pairs are issued before their products
+__device__ __forceinline__ double upd_sum(double acc, const uint32_t* up, int w, int we, const double* Fa,
+                                          const double* Fb) {
+    for (; w + 3 < we; w += 4) {
+        uint32_t ab[4];
+        double fa[4], fb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) ab[u] = up[w + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            fa[u] = Fa[ab[u] & 0xFFFF];
+            fb[u] = Fb[ab[u] >> 16];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc -= fa[u] * fb[u];
+    }
+    for (; w < we; w++) {
+        const uint32_t ab = up[w];
+        acc -= Fa[ab & 0xFFFF] * Fb[ab >> 16];
+    }
+    return acc;
+}
+
 __device__ __forceinline__ void x_task(const uint2* xt, const uint32_t* up, int q, const View& v) {
     const uint2 t = xt[q];
     if ((t.y >> 16) & 2) return;
     const int k = t.x & 0xFFFF, ub = t.x >> 16, ue = t.y & 0xFFFF;
     const double sc = (t.y >> 16) & 1 ? v.inv[k] : 1.0;
-    double acc = v.x[k];
-    for (int w = ub; w < ue; w++) {
-        const uint32_t ab = up[w];
-        acc -= v.F[ab & 0xFFFF] * v.x[ab >> 16];
-    }
+    const double acc = upd_sum(v.x[k], up, ub, ue, v.F, v.x);
     v.x[k] = (t.y >> 16) & 1 ? acc * sc : acc;
 }
 
@@ -366,11 +387,7 @@ __device__ __forceinline__ void factor(const CPlan& c, const View& v) {
                 // the scale is loaded with the operands (its pivot is from an
                 // earlier level), not after the updates
                 const double sc = ctl < CP_PLAIN_U ? inv[ctl] : 1.0;
-                double acc = F[e];
-                for (int w = ub; w < ue; w++) {
-                    const uint32_t ab = up[w];
-                    acc -= F[ab & 0xFFFF] * F[ab >> 16];
-                }
+                const double acc = upd_sum(F[e], up, ub, ue, F, F);
                 if (ctl < CP_PLAIN_U) {
                     F[e] = acc * sc;
                 } else {
